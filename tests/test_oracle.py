@@ -1,0 +1,288 @@
+"""Pins the CPU oracle (oracle/) before anything is compared against it.
+
+Each check is chosen so that a plausible mistake in the oracle fails it: a wrong FFT
+axis or normalisation fails the brute-force DFT fixtures; a wrong sign of the phase
+fails the Gouy-phase and H(-z) = conj H(z) checks; a dropped 2*pi fails the analytic
+Gaussian width; the evanescent rule is checked mode by mode.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+from scipy.special import fresnel
+
+from conftest import rel_l2
+import oracle
+from oracle import profiles as P
+from oracle.angular_spectrum import (OracleBeam2D, construct_grids, transfer_function,
+                                     propagate_field, propagate_field_adjoint, fft2, ifft2,
+                                     nm, um, mm)
+
+
+# ----------------------------------------------------------------- grids / H / FFT
+def test_units():
+    # beam2d.py:37-41
+    assert (oracle.nm, oracle.um, oracle.mm, oracle.cm, oracle.m) == (1e-7, 1e-4, 1e-1, 1, 100)
+
+
+@pytest.mark.parametrize("n", [8, 9, 32])
+def test_grids_closed_form(n):
+    area, wl = 200 * um, 632 * nm
+    g = construct_grids(area, n, wl)
+    dL = area / n
+    assert g.dL == dL
+    assert g.X.shape == (n,) and g.Y.shape == (n, 1)
+    assert g.X[0] == (-n // 2) * dL and np.all(np.diff(g.X) > 0)
+    ks = [i if i < (n + 1) // 2 else i - n for i in range(n)]
+    np.testing.assert_allclose(g.Kx, np.array(ks) / (n * dL), rtol=1e-15)
+    assert g.k0 == 1 / wl
+
+
+def test_kz_evanescent_count():
+    area, n, wl = 8 * um, 32, 632 * nm
+    g = construct_grids(area, n, wl)
+    brute = 0
+    for iy in range(n):
+        for ix in range(n):
+            fx = g.Kx[ix]
+            fy = g.Kx[iy]
+            if fx * fx + fy * fy > g.k0 ** 2:
+                brute += 1
+                assert g.Kz[iy, ix] == 0.0
+            else:
+                assert g.Kz[iy, ix] == pytest.approx(np.sqrt(g.k0 ** 2 - fx * fx - fy * fy), rel=1e-14)
+    assert (g.Kz == 0).sum() == brute > 0
+
+
+def test_transfer_function_properties():
+    area, n, wl = 8 * um, 32, 632 * nm
+    g = construct_grids(area, n, wl)
+    for z in [1 * um, 0.37 * mm, 5 * mm]:
+        h = transfer_function(g.Kz, z)
+        np.testing.assert_allclose(np.abs(h), 1, atol=1e-14)
+        assert np.all(h[g.Kz == 0] == 1)
+        np.testing.assert_allclose(transfer_function(g.Kz, -z), np.conj(h), atol=1e-11)
+        np.testing.assert_allclose(h, np.exp(-2j * np.pi * g.Kz * z), atol=1e-9)
+    h12 = transfer_function(g.Kz, 1 * mm) * transfer_function(g.Kz, 2 * mm)
+    np.testing.assert_allclose(h12, transfer_function(g.Kz, 3 * mm), atol=1e-10)
+
+
+def _dft(n):
+    k = np.arange(n)
+    return np.exp(-2j * np.pi * (np.outer(k, k) % n) / n)
+
+
+@pytest.mark.parametrize("n", [4, 8, 16, 32])
+def test_fft2_brute_force(n):
+    rs = np.random.RandomState(n)
+    a = rs.standard_normal((n, n)) + 1j * rs.standard_normal((n, n))
+    W = _dft(n)
+    assert rel_l2(fft2(a), W @ a @ W.T) < 1e-13
+    assert rel_l2(ifft2(a), np.conj(W) @ a @ np.conj(W).T / n ** 2) < 1e-13
+    assert rel_l2(ifft2(fft2(a)), a) < 1e-14
+
+
+def test_fft2_known_answers():
+    n = 16
+    d = np.zeros((n, n), complex)
+    d[0, 0] = 1
+    np.testing.assert_allclose(fft2(d), np.ones((n, n)))           # delta -> constant
+    x = np.arange(n)
+    tone = np.exp(2j * np.pi * (3 * x[None, :] + 5 * x[:, None]) / n)
+    F = fft2(tone)
+    assert abs(F[5, 3] - n * n) < 1e-9                               # tone -> delta
+    F[5, 3] = 0
+    assert np.abs(F).max() < 1e-9
+
+
+# ----------------------------------------------------------------- golden fixtures
+GOLDEN_CASES = ["gauss_propagate_n32", "speckle_aperture_n64", "evanescent_n32",
+                "lens_chain_n64", "coord_filter_n16"]
+
+
+def load_case(golden_dir, name):
+    z = np.load(os.path.join(golden_dir, name + ".npz"))
+    meta = json.loads(str(z["meta"]))
+    arrays = {k[4:]: z[k] for k in z.files if k.startswith("arr_")}
+    return meta, z, arrays
+
+
+def run_oracle_case(meta, field_in, arrays, complex_bits=128):
+    b = OracleBeam2D(meta["area"], meta["n"], meta["wl"], init_field=field_in.copy(),
+                     complex_bits=complex_bits)
+    for op, arg in meta["ops"]:
+        val = arrays[arg] if isinstance(arg, str) else arg
+        if op == "coordinate_filter":
+            b.coordinate_filter(f_init=val)
+        elif op == "spectral_filter":
+            b.spectral_filter(f_init=val)
+        else:
+            getattr(b, op)(val)
+    return b
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_oracle_matches_golden(golden_dir, name):
+    meta, z, arrays = load_case(golden_dir, name)
+    b = run_oracle_case(meta, z["field_in"], arrays)
+    assert rel_l2(b.field, z["field_out"]) < 1e-10
+    assert rel_l2(b.spectrum, z["spectrum_out"]) < 1e-10
+
+
+@pytest.mark.parametrize("name", ["adjoint_n32", "adjoint_evanescent_n16"])
+def test_oracle_adjoint_matches_golden(golden_dir, name):
+    z = np.load(os.path.join(golden_dir, name + ".npz"))
+    meta = json.loads(str(z["meta"]))
+    fwd = propagate_field(z["u"], meta["area"], meta["wl"], meta["z"], phase_mask=z["phi"])
+    assert rel_l2(fwd, z["fwd"]) < 1e-10
+    gu, gp = propagate_field_adjoint(z["g"], z["u"], meta["area"], meta["wl"], meta["z"],
+                                     phase_mask=z["phi"])
+    assert rel_l2(gu, z["grad_u"]) < 1e-10
+    assert rel_l2(gp, z["grad_phi"]) < 1e-10
+
+
+def test_adjoint_dot_product_and_fd():
+    n, area, wl, zz = 16, 100 * um, 632 * nm, 0.2 * mm
+    rs = np.random.RandomState(7)
+    u = rs.standard_normal((n, n)) + 1j * rs.standard_normal((n, n))
+    g = rs.standard_normal((n, n)) + 1j * rs.standard_normal((n, n))
+    Au = propagate_field(u, area, wl, zz, aperture_radius=0.5 / (2 * area / n))
+    AHg, _ = propagate_field_adjoint(g, u, area, wl, zz, aperture_radius=0.5 / (2 * area / n))
+    assert abs(np.vdot(g, Au) - np.vdot(AHg, u)) < 1e-10 * abs(np.vdot(g, Au))
+    # central finite differences of L = sum |A(u e^{i phi})|^2 * w on a few pixels
+    phi = rs.uniform(0, 2 * np.pi, (n, n))
+    w = rs.uniform(size=(n, n))
+
+    def loss(ph):
+        v = propagate_field(u, area, wl, zz, phase_mask=ph)
+        return float(np.sum(w * np.abs(v) ** 2))
+
+    v = propagate_field(u, area, wl, zz, phase_mask=phi)
+    _, gphi = propagate_field_adjoint(2 * w * v, u, area, wl, zz, phase_mask=phi)
+    h = 1e-6
+    for (iy, ix) in [(0, 0), (3, 7), (15, 2), (8, 8)]:
+        e = np.zeros((n, n))
+        e[iy, ix] = h
+        fd = (loss(phi + e) - loss(phi - e)) / (2 * h)
+        assert fd == pytest.approx(gphi[iy, ix], rel=1e-5, abs=1e-8)
+
+
+# ----------------------------------------------------------------- analytic physics
+def test_gaussian_beam_analytic():
+    """Reference tests/gaussbeam_diffraction.py:51-79 formulas: on-axis intensity
+    1/(1+(z/zR)^2), width w/w0 = sqrt(1+(wl z/(pi R^2))^2) from D4sigma/2, and Gouy
+    phase arctan(wl z/(pi R^2)) after removing 2*pi*(trunc(k0 z) - k0 z)."""
+    npoints, area, wl, R = 256, 2 * mm, 0.532 * um, 0.1 * mm
+    b = OracleBeam2D(area, npoints, wl, init_field_gen=P.gaussian_beam, init_gen_args=(1, R))
+    I0 = b.centroid_intensity
+    zR = np.pi * R ** 2 / wl
+    for z in [2 * mm, 4 * mm, 6 * mm]:
+        b.propagate(2 * mm)
+        zn = z / zR
+        assert b.centroid_intensity / I0 == pytest.approx(1 / (1 + zn ** 2), rel=2e-3)
+        assert (b.D4sigma[0] / 2) / R == pytest.approx(np.sqrt(1 + zn ** 2), rel=2e-3)
+        ph = b.phiprofile[b.centroid[3], b.centroid[2]] - 2 * np.pi * (np.trunc(b.k0 * z) - b.k0 * z)
+        ph = (ph + np.pi) % (2 * np.pi) - np.pi
+        assert ph == pytest.approx(np.arctan(zn), abs=2e-3)
+
+
+def test_config1_gaussian_vs_analytic_field():
+    """256^2 c128, area 200 um, wl 632 nm, rho0 25 um, propagate 100 um: compare with
+    the paraxial Gaussian (R/w)^2 exp(-2 r^2/w^2) intensity (paraxial gap ~1e-4)."""
+    n, area, wl, R = 256, 200 * um, 632 * nm, 25 * um
+    b = OracleBeam2D(area, n, wl, init_field_gen=P.gaussian_beam, init_gen_args=(1, R))
+    z = 100 * um
+    b.propagate(z)
+    zR = np.pi * R ** 2 / wl
+    w = R * np.sqrt(1 + (z / zR) ** 2)
+    r2 = b.X ** 2 + b.Y ** 2
+    theory = (R / w) ** 2 * np.exp(-2 * r2 / w ** 2)
+    assert rel_l2(b.iprofile, theory) < 1e-3
+    assert b.iprofile[n // 2, n // 2] == pytest.approx(1 / (1 + (z / zR) ** 2), rel=1e-4)
+
+
+def test_round_hole_on_axis():
+    """Reference tests/hole_diffraction.py:44-47 formula 4 sin^2(pi a^2/(2 wl z)), with
+    the true radius a = R/sqrt(2) of round_hole(R) (profiles.py:79-80)."""
+    npoints, area, wl, R = 512, 3 * mm, 532e-7, 150 * um
+    b = OracleBeam2D(area, npoints, wl, init_field_gen=P.round_hole, init_gen_args=(R,))
+    a2 = (R / np.sqrt(2)) ** 2
+    # pixelated disc: use its actual area for the effective radius
+    a2 = (b.field.real > 0).sum() * b.dL ** 2 / np.pi
+    for z in [40 * 0.02, 60 * 0.02, 90 * 0.02]:
+        b.propagate(z - b.z)
+        expect = 4 * np.sin(np.pi * a2 / (2 * wl * z)) ** 2
+        assert b.iprofile[npoints // 2, npoints // 2] == pytest.approx(expect, abs=0.15)
+
+
+def test_square_hole_on_axis():
+    """Reference tests/hole_diffraction.py:76-100 (Fresnel integrals)."""
+    npoints, area, wl = 512, 3 * mm, 532e-7
+    d = area / 8
+    b = OracleBeam2D(area, npoints, wl, init_field_gen=P.square_hole, init_gen_args=(d,))
+    deff = np.sqrt((b.field.real > 0).sum()) * b.dL
+    for z in [40 * 0.02, 60 * 0.02]:
+        b.propagate(z - b.z)
+        zn = z * wl * 2 / deff ** 2
+        c1, s1 = fresnel(-np.sqrt(1 / zn))
+        c2, s2 = fresnel(np.sqrt(1 / zn))
+        expect = 0.25 * ((c1 - c2) ** 2 + (s1 - s2) ** 2) ** 2
+        assert b.iprofile[npoints // 2, npoints // 2] == pytest.approx(expect, rel=0.05)
+
+
+def test_propagate_invariants():
+    n, area, wl = 64, 200 * um, 632 * nm
+    rs = np.random.RandomState(11)
+    f = rs.standard_normal((n, n)) + 1j * rs.standard_normal((n, n))
+    a = OracleBeam2D(area, n, wl, init_field=f.copy())
+    b = OracleBeam2D(area, n, wl, init_field=f.copy())
+    a.propagate(0.3 * mm)
+    a.propagate(0.5 * mm)
+    b.propagate(0.8 * mm)
+    assert rel_l2(a.field, b.field) < 1e-10                       # composition
+    a.propagate(-0.8 * mm)
+    assert rel_l2(a.field, f) < 1e-10                             # reversibility
+    n0 = np.sum(np.abs(f) ** 2)
+    assert np.sum(np.abs(b.field) ** 2) == pytest.approx(n0, rel=1e-12)   # Parseval
+    z0 = b.z
+    b.propagate(0)
+    assert b.z == z0                                              # z == 0 early return
+
+
+def test_evanescent_pass_through():
+    n, area, wl = 32, 8 * um, 632 * nm
+    rs = np.random.RandomState(5)
+    f = rs.standard_normal((n, n)) + 1j * rs.standard_normal((n, n))
+    b = OracleBeam2D(area, n, wl, init_field=f)
+    s0 = b.spectrum.copy()
+    b.propagate(3 * um)
+    ev = b.Kz == 0
+    assert ev.sum() > 0
+    np.testing.assert_array_equal(b.spectrum[ev], s0[ev])
+
+
+def test_lens_focuses_with_effective_focal_length():
+    """lens(f) applies exp(i r^2 k0/(2 f)) with k0 = 1/wl (no 2*pi), i.e. a thin lens of
+    focal length 2*pi*f under the exp(-2*pi*i*Kz*z) convention; beyond-focus sign check."""
+    n, area, wl = 256, 2 * mm, 532 * nm
+    b = OracleBeam2D(area, n, wl, init_field_gen=P.gaussian_beam, init_gen_args=(1, 0.4 * mm))
+    f = 1.0 / (2 * np.pi) * 10  # effective focal length 10 cm
+    b.lens(f)
+    b.propagate(10.0)
+    w_focus = b.D4sigma[0] / 2
+    theory = wl * 10.0 / (np.pi * 0.4 * mm)     # focal spot 1/e^2 radius
+    assert w_focus == pytest.approx(theory, rel=0.05)
+
+
+def test_c64_state_close_to_c128():
+    n, area, wl = 64, 400 * um, 632 * nm
+    x = (np.arange(n) - n // 2) * area / n
+    f = P.random_round_hole_phase(x, x.reshape(-1, 1), 100 * um, rng=np.random.RandomState(0))
+    a = OracleBeam2D(area, n, wl, init_field=f.astype(np.complex64), complex_bits=64)
+    b = OracleBeam2D(area, n, wl, init_field=f.astype(np.complex64), complex_bits=128)
+    for _ in range(3):
+        a.propagate(50 * um)
+        b.propagate(50 * um)
+    assert a.field.dtype == np.complex64
+    assert rel_l2(a.field, b.field) < 1e-5
