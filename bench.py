@@ -1,0 +1,178 @@
+"""Benchmark: batched field-propagations/s at 1024^2 complex64 on one MI355X.
+
+A step is one pass of the hot path over one batch of synthetic fields resident in
+HBM: B independent 1024x1024 complex64 fields, each propagated field -> field,
+u_out = F^-1[H(z) F[u_in]] (forward 2-D FFT, transfer function generated on chip,
+inverse FFT).  The forward+adjoint leg times the differentiable step used for
+phase-mask optimisation: forward with a phase screen, then the adjoint with the
+phase-gradient epilogue.
+
+Prints ONE JSON line (rank 0, N = 1).  Per-kernel times come from HIP events recorded
+on the launch stream by liblp2d's tracer; the CPU baseline runs the oracle (a numpy
+restatement of the reference's numpy path) on this host, bounded in time.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+METRIC = ("field-propagations/s at 1024² c64 (fwd & fwd+adjoint), "
+          "% HBM roofline vs CPU")
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=50)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--batch", type=int, default=32)
+    p.add_argument("--n", type=int, default=1024)
+    p.add_argument("--cpu-seconds", type=float, default=12.0,
+                   help="time budget of the CPU baseline sample")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--autotune", action="store_true", help="pick kernel variants first")
+    return p.parse_args()
+
+
+def cpu_baseline(n, budget_s):
+    """Time the oracle's field-propagation (numpy.fft, complex128, one thread: the
+    reference's own numpy fallback, beam2d.py:31-34, 217, 229) on a bounded sample."""
+    import oracle
+    rs = np.random.RandomState(0)
+    area, wl, z = 400e-4, 632e-7, 50e-4
+    u = (rs.standard_normal((n, n)) + 1j * rs.standard_normal((n, n))).astype(np.complex64)
+    oracle.propagate_field(u, area, wl, z)  # warm-up
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        oracle.propagate_field(u, area, wl, z)
+        k += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or k >= 200:
+            break
+    threads = os.environ.get("OMP_NUM_THREADS", "")
+    return {"value": k / el, "unit": "field-propagations/s", "cores": 1, "kind": "port",
+            "sample": f"{k} x {n}x{n} complex128 field-propagations (numpy.fft pocketfft, "
+                      f"single thread; grid + H rebuilt per call as the reference does) "
+                      f"in {el:.1f} s", "omp_num_threads": threads}
+
+
+def trace_stats(lib, L):
+    import ctypes as C
+    arr = (L.lp2d_trace_stat * 64)()
+    cnt = C.c_int()
+    L.check(lib.lp2d_trace_collect(arr, 64, C.byref(cnt)), "trace_collect")
+    return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(cnt.value)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                                   "(batches of independent fields shard by field with no "
+                                   "communication; see DESIGN.md)"}))
+        sys.exit(2)
+    import torch
+    from lightprop2d_amd import _lib as L
+    from lightprop2d_amd import propagation as P
+
+    lib = L.load()
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    n, B = args.n, args.batch
+    area, wl, z = 400e-4, 632e-7, 50e-4
+    if args.autotune:
+        from lightprop2d_amd import autotune
+        autotune.tune(n, "c64", batch=B)
+
+    g = torch.Generator(device="cpu").manual_seed(0)
+    u = torch.randn((B, n, n), dtype=torch.complex64, generator=g).to(dev)
+    out = torch.empty_like(u)
+    phi = (torch.rand((n, n), generator=g) * 2 * np.pi).to(torch.float32).to(dev)
+    gout = torch.randn((B, n, n), dtype=torch.complex64, generator=g).to(dev)
+    gin = torch.empty_like(u)
+
+    def fwd_step():
+        P.propagate(u, z, area, wl, out=out)
+
+    def fwd_adj_step():
+        P.propagate(u, z, area, wl, phase=phi, out=out)
+        P.propagate_adjoint(gout, u, z, area, wl, phase=phi, out=gin)
+
+    def timed(step, K, W):
+        for _ in range(W):
+            step()
+        torch.cuda.synchronize()
+        L.check(lib.lp2d_trace_enable(1))
+        t0 = time.perf_counter()
+        for _ in range(K):
+            step()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        L.check(lib.lp2d_trace_enable(0))
+        return el, trace_stats(lib, L)
+
+    el, kt = timed(fwd_step, args.steps, args.warmup)
+    props = B * args.steps
+    value = props / el
+    el2, kt2 = timed(fwd_adj_step, max(1, args.steps // 2), 2)
+    fa_rate = B * max(1, args.steps // 2) / el2
+
+    # dominant kernel of the forward step: algorithmic bytes per launch = 16 n^2 per
+    # field (each pass reads and writes the field once) x B fields
+    dom = max(kt.items(), key=lambda kv: kv[1][1])
+    dom_name, (dom_launches, dom_ms) = dom
+    avg_s = dom_ms / 1e3 / dom_launches
+    alg_bytes = 16.0 * n * n * B
+    achieved = alg_bytes / avg_s / 1e9
+    e2e = 16.0 * n * n * value / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": dom_name, "kernel_avg_us": round(avg_s * 1e6, 2),
+                "alg_bytes_per_launch": int(alg_bytes),
+                "end_to_end_GBs": round(e2e, 1),
+                "end_to_end_frac": round(e2e / HBM_PEAK_GBS, 4)}
+    kernels = {k: {"launches": c, "avg_us": round(ms * 1e3 / c, 2)} for k, (c, ms) in kt.items()}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(n, args.cpu_seconds)
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "field-propagations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {"workload": f"batched {n}x{n} complex64 field propagation "
+                               f"F^-1[H F u], {B} fields per step, area 400 um, "
+                               f"wl 632 nm, z 50 um, inputs resident in HBM",
+                   "n": n, "batch": B, "parallelism": "single GPU"},
+        "fwd_adjoint": {"value": round(fa_rate, 1), "unit": "field-propagations/s",
+                        "ms_per_step": round(el2 / max(1, args.steps // 2) * 1e3, 4),
+                        "ratio_to_fwd": round(value / fa_rate, 3),
+                        "workload": "forward with phase screen + adjoint with phase gradient"},
+        "roofline": roofline,
+        "kernels": kernels,
+        "kernels_fwd_adjoint": {k: {"launches": c, "avg_us": round(ms * 1e3 / c, 2)}
+                                for k, (c, ms) in kt2.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

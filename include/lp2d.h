@@ -1,0 +1,192 @@
+/*
+ * lp2d.h - C ABI of the MI355X (gfx950) angular-spectrum propagation library.
+ *
+ * This is the seam that replaces lightprop2d's FFT / elementwise backend
+ * (`xp` module + `_fft2` / `_ifft2`, lightprop2d/beam2d.py:105-109, 207-229) and the
+ * operator bodies built on it (propagate :462-490, lens :492-511,
+ * coordinate_filter :310-342, spectral_filter :344-373, iprofile :706-721).
+ * The Python `Beam2D` in lightprop2d_amd/beam2d.py calls these entry points.
+ *
+ * Conventions
+ *  - Every data pointer is a DEVICE pointer owned by the caller.  Fields are
+ *    row-major (batch, n, n) arrays, rows = y, of complex64 (float re, im) or
+ *    complex128 (double re, im), selected by `dtype`.
+ *  - n must be a power of two, 2 <= n <= lp2d_max_n().
+ *  - `stream` is a hipStream_t passed as void* (NULL = default stream).  Calls are
+ *    stream-ordered and asynchronous.  Device memory is allocated only by lp2d_init
+ *    (twiddle tables, once per device) and lp2d_batch_create (work tables).
+ *  - Every call returns LP2D_OK (0) or a negative status; lp2d_last_error() gives
+ *    the message for the calling thread.  No C++ exception crosses the ABI.
+ *  - Forward FFT is unnormalised, inverse scales by 1/n^2 (numpy convention,
+ *    beam2d.py:207-229).
+ *  - Units are the reference's: centimetres (beam2d.py:37-41); k0 = 1/wl.
+ */
+#ifndef LP2D_H
+#define LP2D_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LP2D_OK 0
+#define LP2D_E_ARG (-1)         /* bad argument (null pointer, bad flag)           */
+#define LP2D_E_SIZE (-2)        /* n not a supported power of two, bad batch        */
+#define LP2D_E_HIP (-3)         /* a HIP runtime call failed                         */
+#define LP2D_E_DTYPE (-4)       /* unknown field or mask dtype                       */
+#define LP2D_E_INIT (-5)        /* library not initialised on the current device     */
+
+/* field dtypes */
+#define LP2D_C64 0
+#define LP2D_C128 1
+
+/* mask / multiplier dtypes (lp2d_mask.dtype) */
+#define LP2D_MASK_U8 1          /* bool / uint8                */
+#define LP2D_MASK_F32 2
+#define LP2D_MASK_F64 3
+#define LP2D_MASK_C64 4
+#define LP2D_MASK_C128 5
+
+/* evanescent-mode rule (lp2d_optics.evanescent) */
+#define LP2D_EVANESCENT_PASS 0  /* reference: Kz := 0 so H = 1 (beam2d.py:307-308) */
+#define LP2D_EVANESCENT_ZERO 1  /* opt-in: H = 0                                      */
+#define LP2D_EVANESCENT_DECAY 2 /* opt-in: H = exp(-2*pi*sqrt(f^2 - k0^2)*|z|)        */
+
+/* An elementwise multiplier broadcast over (batch, y, x) with element strides
+ * (0 = broadcast along that axis).  Matches `field *= mask` with numpy broadcasting
+ * of the reference's f_gen(X, Y) / f_init arrays (beam2d.py:338, 341, 369, 372). */
+typedef struct lp2d_mask {
+  const void* ptr;
+  int32_t dtype;
+  int32_t reserved;
+  int64_t stride_b, stride_y, stride_x;
+} lp2d_mask;
+
+/* Optical parameters of one call. */
+typedef struct lp2d_optics {
+  double area_size;       /* cm, grid extent; dL = area_size / n          */
+  double wl;              /* cm, wavelength; k0 = 1/wl                    */
+  double z;               /* cm, propagation distance (0 = no H)          */
+  double aperture_radius; /* cycles/cm; > 0 keeps Kx^2+Ky^2 <= radius^2   */
+  double lens_f;          /* cm; != 0 applies the reference lens phase     */
+  int32_t evanescent;     /* LP2D_EVANESCENT_*                             */
+  int32_t reserved;
+} lp2d_optics;
+
+/* ---------------------------------------------------------------- library */
+/* Prepares the twiddle tables on `device` (idempotent, thread-safe).  Every other
+ * entry point calls it for the current device on first use. */
+int lp2d_init(int device);
+const char* lp2d_last_error(void);
+int lp2d_version(void);
+int lp2d_max_n(int dtype);
+
+/* ---------------------------------------------------------------- grids */
+/* Kz grid, FP64 (n, n): nan_to_num(sqrt(k0^2 - Kx^2 - Ky^2)) with Kx = fftfreq(n, dL).
+ * Replaces beam2d.py:299-308 (the kernels regenerate it on chip; this is for the
+ * `Kz` attribute). */
+int lp2d_kz_grid(int n, const lp2d_optics* optics, double* kz_out, void* stream);
+
+/* ---------------------------------------------------------------- Beam2D state ops */
+/* out = fft2(in) (inverse = 0) or ifft2(in) (inverse = 1); in == out allowed.
+ * Replaces _fft2 / _ifft2 (beam2d.py:207-229). */
+int lp2d_fft2(int n, int batch, int dtype, const void* in, void* out, int inverse,
+              void* stream);
+
+/* field <- field * masks[0] * ... * lens(optics->lens_f) * exp(i*phase);
+ * spectrum_out <- fft2(field) when spectrum_out != NULL.
+ * Replaces coordinate_filter (beam2d.py:335-342) and lens (beam2d.py:508-511).
+ * nmasks <= 2; phase (float32 radians, (n, n) or (batch, n, n) via phase_stride_b)
+ * may be NULL; optics may be NULL when no lens is applied. */
+int lp2d_spatial_apply(int n, int batch, int dtype, void* field, const lp2d_mask* masks,
+                       int nmasks, const lp2d_optics* optics, const float* phase,
+                       int64_t phase_stride_b, void* spectrum_out, void* stream);
+
+/* spectrum <- spectrum * H(optics->z) * aperture * masks[0..nmasks);
+ * field_out <- ifft2(spectrum).  With z != 0 this is propagate (beam2d.py:481-490);
+ * with masks it is spectral_filter (beam2d.py:366-373). */
+int lp2d_spectral_apply(int n, int batch, int dtype, void* spectrum, const lp2d_mask* masks,
+                        int nmasks, const lp2d_optics* optics, void* field_out, void* stream);
+
+/* |field|^2 into float (C64) or double (C128) (beam2d.py:717). */
+int lp2d_intensity(int64_t count, int dtype, const void* field, void* out, void* stream);
+
+/* ---------------------------------------------------------------- field -> field */
+/* field_out = ifft2(H(z) * aperture * fft2(field_in * exp(i*phase) * lens)), fused;
+ * spectrum_out (may be NULL) receives H*aperture*fft2(...), i.e. the reference's
+ * `spectrum` state after the same calls.  field_in == field_out allowed.
+ * Reference sequence: coordinate_filter(f_init=exp(i*phase)) -> lens -> propagate(z)
+ * -> spectral_filter(aperture) (beam2d.py:339-342, 508-511, 481-490, 370-373). */
+int lp2d_propagate(int n, int batch, int dtype, const void* field_in, void* field_out,
+                   const float* phase, int64_t phase_stride_b, const lp2d_optics* optics,
+                   void* spectrum_out, void* stream);
+
+/* Adjoint (vector-Jacobian product) of lp2d_propagate, for a real loss with
+ * PyTorch's complex-gradient convention (grad = dL/dRe + i dL/dIm):
+ *   g1 = ifft2(conj(H) * aperture * fft2(grad_out))
+ *   grad_in = conj(exp(i*phase) * lens) * g1
+ *   grad_phase[b] = Im(conj(field_in[b] * exp(i*phase) * lens) * g1[b])   (per field)
+ * grad_phase (float32, (batch, n, n)) may be NULL, and must be NULL when phase is.
+ * No counterpart in the reference (it has no gradients). */
+int lp2d_propagate_adjoint(int n, int batch, int dtype, const void* grad_out,
+                           const void* field_in, const float* phase, int64_t phase_stride_b,
+                           const lp2d_optics* optics, void* grad_in, float* grad_phase,
+                           void* stream);
+
+/* Sum `batch` float planes of `count` elements: out[i] = sum_b in[b*count + i].
+ * Used to reduce per-field phase gradients of a shared mask. */
+int lp2d_sum_planes(int64_t count, int batch, const float* in, float* out, void* stream);
+
+/* ---------------------------------------------------------------- uneven batches */
+/* One independent field of an uneven batch: its own size, distance and optics. */
+typedef struct lp2d_field_desc {
+  int32_t n;
+  int32_t reserved;
+  const void* src;        /* (n, n) field in  */
+  void* dst;              /* (n, n) field out (may equal src) */
+  double area_size, wl, z, aperture_radius;
+} lp2d_field_desc;
+
+typedef struct lp2d_batch lp2d_batch;
+
+/* Builds the size-bucketed work tables of an uneven batch (allocates and uploads
+ * them; call outside graph capture).  dtype applies to all fields. */
+int lp2d_batch_create(const lp2d_field_desc* descs, int count, int dtype, lp2d_batch** out);
+/* Runs every field-propagation of the batch: three persistent launches (the
+ * single-tile sizes finish in the first).  Allocates nothing, so it can be
+ * captured in a hipGraph. */
+int lp2d_batch_run(lp2d_batch* batch, void* stream);
+int lp2d_batch_destroy(lp2d_batch* batch);
+/* Workgroups per persistent launch (0 = one per CU times occupancy). */
+int lp2d_batch_set_grid(lp2d_batch* batch, int workgroups);
+
+/* ---------------------------------------------------------------- tracing */
+/* Kernel timing on the launch stream: while enabled, every kernel launch of the
+ * library is bracketed by a pair of HIP events recorded on the stream it runs on.
+ * lp2d_trace_collect waits for them, sums the elapsed time per kernel name and
+ * clears the record.  Used by bench.py for per-kernel roofline figures.  No
+ * counterpart in the reference. */
+typedef struct lp2d_trace_stat {
+  char name[64];
+  int32_t launches;
+  int32_t reserved;
+  double total_ms;
+} lp2d_trace_stat;
+int lp2d_trace_enable(int on);
+int lp2d_trace_collect(lp2d_trace_stat* stats, int max_stats, int* n_stats);
+
+/* ---------------------------------------------------------------- tuning */
+/* Number of kernel variants (schedules) available for (n, dtype, kind), kind 0 =
+ * row pass, 1 = column pass; and the one in use.  Used by the autotuner. */
+int lp2d_variant_count(int n, int dtype, int kind);
+int lp2d_set_variant(int n, int dtype, int kind, int variant);
+int lp2d_get_variant(int n, int dtype, int kind);
+/* Describes a variant as "E=16 W=16 ..." into buf (returns length). */
+int lp2d_variant_name(int n, int dtype, int kind, int variant, char* buf, int buflen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LP2D_H */

@@ -1,0 +1,140 @@
+"""ctypes binding of liblp2d.so (include/lp2d.h).
+
+The library is built in-tree by ``lightprop2d_amd._build`` and loaded from the package
+directory.  There is no fallback: if the library is missing or cannot be loaded,
+every operation raises ``Lp2dError``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "liblp2d.so")
+
+LP2D_OK = 0
+LP2D_E_ARG = -1
+LP2D_E_SIZE = -2
+LP2D_E_HIP = -3
+LP2D_E_DTYPE = -4
+LP2D_E_INIT = -5
+
+LP2D_C64 = 0
+LP2D_C128 = 1
+
+LP2D_MASK_U8 = 1
+LP2D_MASK_F32 = 2
+LP2D_MASK_F64 = 3
+LP2D_MASK_C64 = 4
+LP2D_MASK_C128 = 5
+
+EVANESCENT = {"reference": 0, "pass": 0, "zero": 1, "decay": 2}
+
+# every symbol include/lp2d.h declares
+EXPORTED = [
+    "lp2d_init", "lp2d_last_error", "lp2d_version", "lp2d_max_n", "lp2d_kz_grid",
+    "lp2d_fft2", "lp2d_spatial_apply", "lp2d_spectral_apply", "lp2d_intensity",
+    "lp2d_propagate", "lp2d_propagate_adjoint", "lp2d_sum_planes", "lp2d_batch_create",
+    "lp2d_batch_run", "lp2d_batch_destroy", "lp2d_batch_set_grid", "lp2d_variant_count",
+    "lp2d_set_variant", "lp2d_get_variant", "lp2d_variant_name", "lp2d_trace_enable",
+    "lp2d_trace_collect",
+]
+
+
+class Lp2dError(RuntimeError):
+    """A liblp2d call failed (negative status) or the library is unavailable."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
+
+
+class lp2d_mask(C.Structure):
+    _fields_ = [("ptr", C.c_void_p), ("dtype", C.c_int32), ("reserved", C.c_int32),
+                ("stride_b", C.c_int64), ("stride_y", C.c_int64), ("stride_x", C.c_int64)]
+
+
+class lp2d_optics(C.Structure):
+    _fields_ = [("area_size", C.c_double), ("wl", C.c_double), ("z", C.c_double),
+                ("aperture_radius", C.c_double), ("lens_f", C.c_double),
+                ("evanescent", C.c_int32), ("reserved", C.c_int32)]
+
+
+class lp2d_field_desc(C.Structure):
+    _fields_ = [("n", C.c_int32), ("reserved", C.c_int32), ("src", C.c_void_p),
+                ("dst", C.c_void_p), ("area_size", C.c_double), ("wl", C.c_double),
+                ("z", C.c_double), ("aperture_radius", C.c_double)]
+
+
+class lp2d_trace_stat(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("launches", C.c_int32), ("reserved", C.c_int32),
+                ("total_ms", C.c_double)]
+
+
+_lib = None
+_load_error = None
+
+_P = C.c_void_p
+_I = C.c_int
+_I64 = C.c_int64
+
+_SIGS = {
+    "lp2d_init": (_I, [_I]),
+    "lp2d_last_error": (C.c_char_p, []),
+    "lp2d_version": (_I, []),
+    "lp2d_max_n": (_I, [_I]),
+    "lp2d_kz_grid": (_I, [_I, C.POINTER(lp2d_optics), _P, _P]),
+    "lp2d_fft2": (_I, [_I, _I, _I, _P, _P, _I, _P]),
+    "lp2d_spatial_apply": (_I, [_I, _I, _I, _P, C.POINTER(lp2d_mask), _I,
+                                C.POINTER(lp2d_optics), _P, _I64, _P, _P]),
+    "lp2d_spectral_apply": (_I, [_I, _I, _I, _P, C.POINTER(lp2d_mask), _I,
+                                 C.POINTER(lp2d_optics), _P, _P]),
+    "lp2d_intensity": (_I, [_I64, _I, _P, _P, _P]),
+    "lp2d_propagate": (_I, [_I, _I, _I, _P, _P, _P, _I64, C.POINTER(lp2d_optics), _P, _P]),
+    "lp2d_propagate_adjoint": (_I, [_I, _I, _I, _P, _P, _P, _I64, C.POINTER(lp2d_optics),
+                                    _P, _P, _P]),
+    "lp2d_sum_planes": (_I, [_I64, _I, _P, _P, _P]),
+    "lp2d_batch_create": (_I, [C.POINTER(lp2d_field_desc), _I, _I, C.POINTER(_P)]),
+    "lp2d_batch_run": (_I, [_P, _P]),
+    "lp2d_batch_destroy": (_I, [_P]),
+    "lp2d_batch_set_grid": (_I, [_P, _I]),
+    "lp2d_variant_count": (_I, [_I, _I, _I]),
+    "lp2d_set_variant": (_I, [_I, _I, _I, _I]),
+    "lp2d_get_variant": (_I, [_I, _I, _I]),
+    "lp2d_variant_name": (_I, [_I, _I, _I, _I, C.c_char_p, _I]),
+    "lp2d_trace_enable": (_I, [_I]),
+    "lp2d_trace_collect": (_I, [C.POINTER(lp2d_trace_stat), _I, C.POINTER(_I)]),
+}
+
+
+def load(path: str | None = None):
+    """Load liblp2d.so (once).  Raises Lp2dError when it is missing."""
+    global _lib, _load_error
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        _load_error = (f"{p} not found: build it with `python -m lightprop2d_amd._build` "
+                       "(the HIP library is required; there is no CPU fallback)")
+        raise Lp2dError(_load_error)
+    try:
+        lib = C.CDLL(p)
+    except OSError as e:  # pragma: no cover - depends on the ROCm runtime
+        _load_error = f"cannot load {p}: {e}"
+        raise Lp2dError(_load_error) from e
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != LP2D_OK:
+        msg = _lib.lp2d_last_error().decode() if _lib is not None else ""
+        raise Lp2dError(f"{what}: liblp2d status {rc}: {msg}".strip(": "), rc)
+
+
+def lib():
+    return load()

@@ -1,0 +1,757 @@
+// C ABI of liblp2d (declared in include/lp2d.h): argument checking, kernel-sequence
+// selection for each operation, twiddle tables, and the small elementwise kernels.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/lp2d.h"
+#include "lp2d_registry.hpp"
+
+using namespace lp2d;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(LP2D_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+constexpr int kMaxDevices = 64;
+struct DeviceTables {
+  cx<float>* twf = nullptr;
+  cx<double>* twd = nullptr;
+  bool ready = false;
+};
+std::mutex g_mu;
+DeviceTables g_dev[kMaxDevices];
+
+Registry<float>* g_rf = nullptr;
+Registry<double>* g_rd = nullptr;
+int g_sel[2][2][kLogMax + 1];  // [dtype][kind][log2 n] -> selected variant
+
+void ensure_registry() {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    g_rf = new Registry<float>();
+    g_rd = new Registry<double>();
+    std::memset(g_rf, 0, sizeof(Registry<float>));
+    std::memset(g_rd, 0, sizeof(Registry<double>));
+    register_f32_a(*g_rf);
+    register_f32_b(*g_rf);
+    register_f64_a(*g_rd);
+    register_f64_b(*g_rd);
+    std::memset(g_sel, 0, sizeof(g_sel));
+  });
+}
+
+__global__ void k_init_twiddles(cx<float>* tf, cx<double>* td) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= kTwiddleLen - 2) return;
+  int N = 2;
+  while (e >= 2 * N - 2) N *= 2;
+  const int k = e - (N - 2);
+  double s, c;
+  sincospi(-2.0 * (double)k / (double)N, &s, &c);
+  td[e] = mk(c, s);
+  tf[e] = mk((float)c, (float)s);
+}
+
+int init_device(int device) {
+  ensure_registry();
+  if (device < 0 || device >= kMaxDevices) return fail(LP2D_E_ARG, "device %d out of range", device);
+  std::lock_guard<std::mutex> lk(g_mu);
+  DeviceTables& d = g_dev[device];
+  if (d.ready) return LP2D_OK;
+  int prev = 0;
+  hipError_t e = hipGetDevice(&prev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (prev != device && (e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+  hipStream_t s = nullptr;
+  int rc = LP2D_OK;
+  if ((e = hipMalloc(&d.twf, kTwiddleLen * sizeof(cx<float>))) != hipSuccess ||
+      (e = hipMalloc(&d.twd, kTwiddleLen * sizeof(cx<double>))) != hipSuccess ||
+      (e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess) {
+    rc = hip_fail(e, "twiddle table allocation");
+  } else {
+    k_init_twiddles<<<(kTwiddleLen + 255) / 256, 256, 0, s>>>(d.twf, d.twd);
+    if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(s)) != hipSuccess)
+      rc = hip_fail(e, "twiddle table init");
+    else
+      d.ready = true;
+  }
+  if (s) hipStreamDestroy(s);
+  if (prev != device) hipSetDevice(prev);
+  return rc;
+}
+
+int current_tables(DeviceTables** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return fail(LP2D_E_ARG, "device %d out of range", dev);
+  if (!g_dev[dev].ready) {
+    int rc = init_device(dev);
+    if (rc) return rc;
+  }
+  *out = &g_dev[dev];
+  return LP2D_OK;
+}
+
+int log2_exact(int n) {
+  if (n < 2 || (n & (n - 1))) return -1;
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return l;
+}
+
+int check_n(int n, int batch, int dtype, int* lg) {
+  if (dtype != LP2D_C64 && dtype != LP2D_C128) return fail(LP2D_E_DTYPE, "unknown dtype %d", dtype);
+  *lg = log2_exact(n);
+  if (*lg < 1 || *lg > kLogMax)
+    return fail(LP2D_E_SIZE, "n=%d: must be a power of two in [2, %d]", n, kMaxN);
+  if (batch < 1) return fail(LP2D_E_SIZE, "batch=%d must be >= 1", batch);
+  if ((long long)batch * n > (1LL << 31) / 2)
+    return fail(LP2D_E_SIZE, "batch*n too large for one launch");
+  return LP2D_OK;
+}
+
+int mask_dtype_ok(int t) { return t >= LP2D_MASK_U8 && t <= LP2D_MASK_C128; }
+
+MaskD to_mask(const lp2d_mask& m) {
+  MaskD d;
+  d.ptr = m.ptr;
+  d.dtype = m.dtype;
+  d.pad_ = 0;
+  d.sb = m.stride_b;
+  d.sy = m.stride_y;
+  d.sx = m.stride_x;
+  return d;
+}
+
+template <typename T>
+PassArgs<T> base_args(const DeviceTables* t, int n, int batch) {
+  PassArgs<T> a;
+  std::memset(&a, 0, sizeof(a));
+  if constexpr (sizeof(T) == 4)
+    a.tw = t->twf;
+  else
+    a.tw = t->twd;
+  a.fstride = (long long)n * n;
+  a.batch = batch;
+  a.scale = (T)1;
+  return a;
+}
+
+template <typename T>
+void set_optics(PassArgs<T>& a, int n, const lp2d_optics* o) {
+  const double dL = o->area_size / n;
+  const double k0 = 1.0 / o->wl;
+  a.dL = dL;
+  a.k0sq = k0 * k0;
+  a.inv_ndL = 1.0 / ((double)n * dL);  // numpy.fft.fftfreq: 1.0 / (n * d)
+  a.evan = o->evanescent;
+  a.lens_rev = (o->lens_f != 0.0) ? k0 / (2.0 * o->lens_f) * 0.15915494309189533577 : 0.0;
+  a.ap_r2 = o->aperture_radius * o->aperture_radius;
+  a.z = o->z;
+}
+
+int check_optics(const lp2d_optics* o) {
+  if (!o) return LP2D_OK;
+  if (!(o->area_size > 0) || !(o->wl > 0)) return fail(LP2D_E_ARG, "area_size and wl must be > 0");
+  if (o->evanescent < 0 || o->evanescent > 2) return fail(LP2D_E_ARG, "bad evanescent mode");
+  return LP2D_OK;
+}
+
+template <typename T>
+Registry<T>* reg();
+template <>
+Registry<float>* reg<float>() { return g_rf; }
+template <>
+Registry<double>* reg<double>() { return g_rd; }
+
+template <typename T>
+constexpr int dt() { return sizeof(T) == 4 ? 0 : 1; }
+
+template <typename T>
+bool use_small(int lg) { return reg<T>()->has_small[lg]; }
+
+// ----------------------------------------------------------------- tracing
+struct TraceRec {
+  std::string name;
+  hipEvent_t start, stop;
+};
+std::mutex g_trace_mu;
+bool g_trace_on = false;
+std::vector<TraceRec> g_trace;
+std::vector<hipEvent_t> g_event_pool;
+
+hipEvent_t pool_event() {
+  if (!g_event_pool.empty()) {
+    hipEvent_t e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  hipEventCreate(&e);
+  return e;
+}
+
+struct TraceScope {
+  hipStream_t s;
+  bool on;
+  TraceRec rec;
+  TraceScope(hipStream_t st, const std::string& name) : s(st), on(false) {
+    if (!g_trace_on) return;
+    std::lock_guard<std::mutex> lk(g_trace_mu);
+    rec.name = name;
+    rec.start = pool_event();
+    rec.stop = pool_event();
+    on = rec.start && rec.stop && hipEventRecord(rec.start, s) == hipSuccess;
+  }
+  ~TraceScope() {
+    if (!on) return;
+    std::lock_guard<std::mutex> lk(g_trace_mu);
+    hipEventRecord(rec.stop, s);
+    g_trace.push_back(rec);
+  }
+};
+
+template <typename T>
+const char* tname() { return sizeof(T) == 4 ? "c64" : "c128"; }
+
+template <typename T>
+int run_rows(const PassArgs<T>& a, int lg, bool inv, hipStream_t s) {
+  const RowVar<T>& v = reg<T>()->rows[lg][g_sel[dt<T>()][0][lg]];
+  std::string nm;
+  if (g_trace_on) {
+    char b[64];
+    snprintf(b, sizeof(b), "rows_%s n=%d %s E=%d", inv ? "inv" : "fwd", 1 << lg, tname<T>(), v.E);
+    nm = b;
+  }
+  TraceScope ts(s, nm);
+  hipError_t e = v.fn(a, inv, s);
+  return e == hipSuccess ? LP2D_OK : hip_fail(e, "row pass launch");
+}
+template <typename T>
+int run_cols(const PassArgs<T>& a, int lg, int mode, hipStream_t s) {
+  const ColVar<T>& v = reg<T>()->cols[lg][g_sel[dt<T>()][1][lg]];
+  std::string nm;
+  if (g_trace_on) {
+    char b[64];
+    snprintf(b, sizeof(b), "cols_m%d n=%d %s E=%d W=%d", mode, 1 << lg, tname<T>(), v.E, v.W);
+    nm = b;
+  }
+  TraceScope ts(s, nm);
+  hipError_t e = v.fn(a, mode, s);
+  return e == hipSuccess ? LP2D_OK : hip_fail(e, "column pass launch");
+}
+template <typename T>
+int run_small(const PassArgs<T>& a, int lg, int mode, hipStream_t s) {
+  std::string nm;
+  if (g_trace_on) {
+    char b[64];
+    snprintf(b, sizeof(b), "field_m%d n=%d %s", mode, 1 << lg, tname<T>());
+    nm = b;
+  }
+  TraceScope ts(s, nm);
+  hipError_t e = reg<T>()->small[lg].fn(a, mode, s);
+  return e == hipSuccess ? LP2D_OK : hip_fail(e, "whole-field launch");
+}
+
+// ----------------------------------------------------------------- elementwise kernels
+template <typename T>
+__global__ void k_spatial_only(PassArgs<T> a, int n, long long total) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int b = (int)(i / a.fstride);
+    const long long r = i - (long long)b * a.fstride;
+    const int y = (int)(r / n), x = (int)(r - (long long)y * n);
+    a.field_out[i] = spatial_pro(a, a.src[i], b, y, x, n);
+  }
+}
+
+template <typename T>
+__global__ void k_spectral_only(PassArgs<T> a, int n, long long total) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int b = (int)(i / a.fstride);
+    const long long r = i - (long long)b * a.fstride;
+    const int ky = (int)(r / n), kx = (int)(r - (long long)ky * n);
+    a.spec_out[i] = spectral_mid(a, a.src[i], b, ky, kx, n);
+  }
+}
+
+template <typename T>
+__global__ void k_intensity(const cx<T>* __restrict__ f, T* __restrict__ out, long long count) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count;
+       i += (long long)gridDim.x * blockDim.x) {
+    const cx<T> v = f[i];
+    out[i] = v.re * v.re + v.im * v.im;
+  }
+}
+
+__global__ void k_sum_planes(const float* __restrict__ in, float* __restrict__ out, long long count,
+                             int batch) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count;
+       i += (long long)gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int b = 0; b < batch; ++b) s += in[(long long)b * count + i];
+    out[i] = s;
+  }
+}
+
+__global__ void k_kz(double* __restrict__ kz, int n, double k0sq, double inv_ndL) {
+#pragma clang fp contract(off)
+  const long long total = (long long)n * n;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int ky = (int)(i / n), kx = (int)(i - (long long)ky * n);
+    const int sx = kx < (n + 1) / 2 ? kx : kx - n;
+    const int sy = ky < (n + 1) / 2 ? ky : ky - n;
+    const double fx = (double)sx * inv_ndL, fy = (double)sy * inv_ndL;
+    const double kz2 = (k0sq - fx * fx) - fy * fy;
+    kz[i] = kz2 >= 0.0 ? sqrt(kz2) : 0.0;
+  }
+}
+
+unsigned grid_for(long long total) {
+  long long g = (total + 255) / 256;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+// ----------------------------------------------------------------- operations
+template <typename T>
+int op_fft2(int n, int lg, int batch, const void* in, void* out, int inverse, hipStream_t s) {
+  DeviceTables* t;
+  if (int rc = current_tables(&t)) return rc;
+  PassArgs<T> a = base_args<T>(t, n, batch);
+  if (use_small<T>(lg)) {
+    a.src = (const cx<T>*)in;
+    a.dst = (cx<T>*)out;
+    if (inverse) {
+      a.flags = F_SCALE;
+      a.scale = (T)(1.0 / ((double)n * n));
+    }
+    return run_small<T>(a, lg, inverse ? MODE_INV : MODE_FWD, s);
+  }
+  if (!inverse) {
+    a.src = (const cx<T>*)in;
+    a.dst = (cx<T>*)out;
+    if (int rc = run_rows<T>(a, lg, false, s)) return rc;
+    a.src = (const cx<T>*)out;
+    return run_cols<T>(a, lg, MODE_FWD, s);
+  }
+  a.src = (const cx<T>*)in;
+  a.dst = (cx<T>*)out;
+  if (int rc = run_cols<T>(a, lg, MODE_INV, s)) return rc;
+  a.src = (const cx<T>*)out;
+  a.flags = F_SCALE;
+  a.scale = (T)(1.0 / ((double)n * n));
+  return run_rows<T>(a, lg, true, s);
+}
+
+template <typename T>
+int op_spatial(int n, int lg, int batch, void* field, const lp2d_mask* masks, int nmasks,
+               const lp2d_optics* o, const float* phase, int64_t phase_sb, void* spec,
+               hipStream_t s) {
+  DeviceTables* t;
+  if (int rc = current_tables(&t)) return rc;
+  PassArgs<T> a = base_args<T>(t, n, batch);
+  int fl = 0;
+  if (nmasks > 0) {
+    a.m0 = to_mask(masks[0]);
+    fl |= F_MASK0;
+  }
+  if (nmasks > 1) {
+    a.m1 = to_mask(masks[1]);
+    fl |= F_MASK1;
+  }
+  if (o) {
+    set_optics(a, n, o);
+    if (o->lens_f != 0.0) fl |= F_LENS;
+  }
+  if (phase) {
+    a.phase = phase;
+    a.phase_sb = phase_sb;
+    fl |= F_PHASE;
+  }
+  a.field_out = (cx<T>*)field;
+  fl |= F_FIELD_OUT;
+  a.src = (const cx<T>*)field;
+  if (!spec) {
+    a.flags = fl;
+    const long long total = (long long)batch * n * n;
+    k_spatial_only<T><<<grid_for(total), 256, 0, s>>>(a, n, total);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? LP2D_OK : hip_fail(e, "spatial kernel launch");
+  }
+  a.dst = (cx<T>*)spec;
+  a.flags = fl;
+  if (use_small<T>(lg)) return run_small<T>(a, lg, MODE_FWD, s);
+  if (int rc = run_rows<T>(a, lg, false, s)) return rc;
+  a.flags = 0;
+  a.src = (const cx<T>*)spec;
+  return run_cols<T>(a, lg, MODE_FWD, s);
+}
+
+template <typename T>
+int op_spectral(int n, int lg, int batch, void* spec, const lp2d_mask* masks, int nmasks,
+                const lp2d_optics* o, void* field_out, hipStream_t s) {
+  DeviceTables* t;
+  if (int rc = current_tables(&t)) return rc;
+  PassArgs<T> a = base_args<T>(t, n, batch);
+  int fl = F_SPEC_OUT;
+  if (o) {
+    set_optics(a, n, o);
+    if (o->z != 0.0) fl |= F_H;
+    if (o->aperture_radius > 0.0) fl |= F_APERTURE;
+  }
+  if (nmasks > 0) {
+    a.s0 = to_mask(masks[0]);
+    fl |= F_SMASK0;
+  }
+  if (nmasks > 1) {
+    a.s1 = to_mask(masks[1]);
+    fl |= F_SMASK1;
+  }
+  a.src = (const cx<T>*)spec;
+  a.spec_out = (cx<T>*)spec;
+  if (!field_out) {
+    a.flags = fl;
+    const long long total = (long long)batch * n * n;
+    k_spectral_only<T><<<grid_for(total), 256, 0, s>>>(a, n, total);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? LP2D_OK : hip_fail(e, "spectral kernel launch");
+  }
+  a.dst = (cx<T>*)field_out;
+  const T sc = (T)(1.0 / ((double)n * n));
+  if (use_small<T>(lg)) {
+    a.flags = fl | F_SCALE;
+    a.scale = sc;
+    return run_small<T>(a, lg, MODE_INV, s);
+  }
+  a.flags = fl;
+  if (int rc = run_cols<T>(a, lg, MODE_INV, s)) return rc;
+  PassArgs<T> r = base_args<T>(t, n, batch);
+  r.src = (const cx<T>*)field_out;
+  r.dst = (cx<T>*)field_out;
+  r.flags = F_SCALE;
+  r.scale = sc;
+  return run_rows<T>(r, lg, true, s);
+}
+
+template <typename T>
+int op_propagate(int n, int lg, int batch, const void* in, void* out, const float* phase,
+                 int64_t phase_sb, const lp2d_optics* o, void* spec, hipStream_t s) {
+  DeviceTables* t;
+  if (int rc = current_tables(&t)) return rc;
+  PassArgs<T> a = base_args<T>(t, n, batch);
+  set_optics(a, n, o);
+  int pro = 0, mid = 0;
+  if (phase) {
+    a.phase = phase;
+    a.phase_sb = phase_sb;
+    pro |= F_PHASE;
+  }
+  if (o->lens_f != 0.0) pro |= F_LENS;
+  if (o->z != 0.0) mid |= F_H;
+  if (o->aperture_radius > 0.0) mid |= F_APERTURE;
+  if (spec) {
+    a.spec_out = (cx<T>*)spec;
+    mid |= F_SPEC_OUT;
+  }
+  const T sc = (T)(1.0 / ((double)n * n));
+  a.src = (const cx<T>*)in;
+  a.dst = (cx<T>*)out;
+  if (use_small<T>(lg)) {
+    a.flags = pro | mid | F_SCALE;
+    a.scale = sc;
+    return run_small<T>(a, lg, MODE_FWD | MODE_INV, s);
+  }
+  a.flags = pro;
+  if (int rc = run_rows<T>(a, lg, false, s)) return rc;
+  a.src = (const cx<T>*)out;
+  a.flags = mid;
+  if (int rc = run_cols<T>(a, lg, MODE_FWD | MODE_INV, s)) return rc;
+  a.flags = F_SCALE;
+  a.scale = sc;
+  return run_rows<T>(a, lg, true, s);
+}
+
+template <typename T>
+int op_adjoint(int n, int lg, int batch, const void* gout, const void* u, const float* phase,
+               int64_t phase_sb, const lp2d_optics* o, void* gin, float* gphase, hipStream_t s) {
+  DeviceTables* t;
+  if (int rc = current_tables(&t)) return rc;
+  PassArgs<T> a = base_args<T>(t, n, batch);
+  set_optics(a, n, o);
+  a.z = -o->z;  // conj(H(z)) = H(-z) under the reference's trunc rule; evanescent stays 1
+  int mid = 0, epi = F_SCALE;
+  if (o->z != 0.0) mid |= F_H;
+  if (o->aperture_radius > 0.0) mid |= F_APERTURE;
+  if (phase || o->lens_f != 0.0) {
+    a.phase = phase;
+    a.phase_sb = phase_sb;
+    a.u_saved = (const cx<T>*)u;
+    a.u_sb = (long long)n * n;
+    a.grad_phase = gphase;
+    epi |= F_GRAD;
+  }
+  const T sc = (T)(1.0 / ((double)n * n));
+  a.scale = sc;
+  a.src = (const cx<T>*)gout;
+  a.dst = (cx<T>*)gin;
+  if (use_small<T>(lg)) {
+    a.flags = mid | epi;
+    return run_small<T>(a, lg, MODE_FWD | MODE_INV, s);
+  }
+  a.flags = 0;
+  if (int rc = run_rows<T>(a, lg, false, s)) return rc;
+  a.src = (const cx<T>*)gin;
+  a.flags = mid;
+  if (int rc = run_cols<T>(a, lg, MODE_FWD | MODE_INV, s)) return rc;
+  a.flags = epi;
+  return run_rows<T>(a, lg, true, s);
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+extern "C" {
+
+int lp2d_init(int device) { return init_device(device); }
+
+const char* lp2d_last_error(void) { return g_err.c_str(); }
+
+int lp2d_version(void) { return 10000; }
+
+int lp2d_max_n(int dtype) { return dtype == LP2D_C64 || dtype == LP2D_C128 ? kMaxN : 0; }
+
+int lp2d_kz_grid(int n, const lp2d_optics* o, double* kz, void* stream) {
+  int lg;
+  if (int rc = check_n(n, 1, LP2D_C64, &lg)) return rc;
+  if (!o || !kz) return fail(LP2D_E_ARG, "null optics or output");
+  if (int rc = check_optics(o)) return rc;
+  DeviceTables* t;
+  if (int rc = current_tables(&t)) return rc;
+  const double dL = o->area_size / n, k0 = 1.0 / o->wl;
+  k_kz<<<grid_for((long long)n * n), 256, 0, (hipStream_t)stream>>>(kz, n, k0 * k0,
+                                                                      1.0 / ((double)n * dL));
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? LP2D_OK : hip_fail(e, "kz kernel launch");
+}
+
+int lp2d_fft2(int n, int batch, int dtype, const void* in, void* out, int inverse, void* stream) {
+  int lg;
+  if (int rc = check_n(n, batch, dtype, &lg)) return rc;
+  if (!in || !out) return fail(LP2D_E_ARG, "null field pointer");
+  hipStream_t s = (hipStream_t)stream;
+  return dtype == LP2D_C64 ? op_fft2<float>(n, lg, batch, in, out, inverse, s)
+                           : op_fft2<double>(n, lg, batch, in, out, inverse, s);
+}
+
+int lp2d_spatial_apply(int n, int batch, int dtype, void* field, const lp2d_mask* masks, int nmasks,
+                       const lp2d_optics* optics, const float* phase, int64_t phase_stride_b,
+                       void* spectrum_out, void* stream) {
+  int lg;
+  if (int rc = check_n(n, batch, dtype, &lg)) return rc;
+  if (!field) return fail(LP2D_E_ARG, "null field pointer");
+  if (nmasks < 0 || nmasks > 2 || (nmasks > 0 && !masks)) return fail(LP2D_E_ARG, "nmasks must be 0..2");
+  for (int i = 0; i < nmasks; ++i) {
+    if (!masks[i].ptr) return fail(LP2D_E_ARG, "mask %d has a null pointer", i);
+    if (!mask_dtype_ok(masks[i].dtype)) return fail(LP2D_E_DTYPE, "mask %d: unknown dtype %d", i, masks[i].dtype);
+  }
+  if (int rc = check_optics(optics)) return rc;
+  if (spectrum_out == field) return fail(LP2D_E_ARG, "spectrum_out must not alias field");
+  hipStream_t s = (hipStream_t)stream;
+  return dtype == LP2D_C64
+             ? op_spatial<float>(n, lg, batch, field, masks, nmasks, optics, phase, phase_stride_b, spectrum_out, s)
+             : op_spatial<double>(n, lg, batch, field, masks, nmasks, optics, phase, phase_stride_b, spectrum_out, s);
+}
+
+int lp2d_spectral_apply(int n, int batch, int dtype, void* spectrum, const lp2d_mask* masks,
+                        int nmasks, const lp2d_optics* optics, void* field_out, void* stream) {
+  int lg;
+  if (int rc = check_n(n, batch, dtype, &lg)) return rc;
+  if (!spectrum) return fail(LP2D_E_ARG, "null spectrum pointer");
+  if (nmasks < 0 || nmasks > 2 || (nmasks > 0 && !masks)) return fail(LP2D_E_ARG, "nmasks must be 0..2");
+  for (int i = 0; i < nmasks; ++i) {
+    if (!masks[i].ptr) return fail(LP2D_E_ARG, "mask %d has a null pointer", i);
+    if (!mask_dtype_ok(masks[i].dtype)) return fail(LP2D_E_DTYPE, "mask %d: unknown dtype %d", i, masks[i].dtype);
+  }
+  if (int rc = check_optics(optics)) return rc;
+  if (field_out == spectrum) return fail(LP2D_E_ARG, "field_out must not alias spectrum");
+  hipStream_t s = (hipStream_t)stream;
+  return dtype == LP2D_C64 ? op_spectral<float>(n, lg, batch, spectrum, masks, nmasks, optics, field_out, s)
+                           : op_spectral<double>(n, lg, batch, spectrum, masks, nmasks, optics, field_out, s);
+}
+
+int lp2d_intensity(int64_t count, int dtype, const void* field, void* out, void* stream) {
+  if (count < 0) return fail(LP2D_E_SIZE, "negative count");
+  if (!field || !out) return fail(LP2D_E_ARG, "null pointer");
+  if (count == 0) return LP2D_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == LP2D_C64)
+    k_intensity<float><<<grid_for(count), 256, 0, s>>>((const cx<float>*)field, (float*)out, count);
+  else if (dtype == LP2D_C128)
+    k_intensity<double><<<grid_for(count), 256, 0, s>>>((const cx<double>*)field, (double*)out, count);
+  else
+    return fail(LP2D_E_DTYPE, "unknown dtype %d", dtype);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? LP2D_OK : hip_fail(e, "intensity kernel launch");
+}
+
+int lp2d_propagate(int n, int batch, int dtype, const void* field_in, void* field_out,
+                   const float* phase, int64_t phase_stride_b, const lp2d_optics* optics,
+                   void* spectrum_out, void* stream) {
+  int lg;
+  if (int rc = check_n(n, batch, dtype, &lg)) return rc;
+  if (!field_in || !field_out || !optics) return fail(LP2D_E_ARG, "null pointer");
+  if (int rc = check_optics(optics)) return rc;
+  if (spectrum_out && (spectrum_out == field_out || spectrum_out == field_in))
+    return fail(LP2D_E_ARG, "spectrum_out must not alias the fields");
+  hipStream_t s = (hipStream_t)stream;
+  return dtype == LP2D_C64
+             ? op_propagate<float>(n, lg, batch, field_in, field_out, phase, phase_stride_b, optics, spectrum_out, s)
+             : op_propagate<double>(n, lg, batch, field_in, field_out, phase, phase_stride_b, optics, spectrum_out, s);
+}
+
+int lp2d_propagate_adjoint(int n, int batch, int dtype, const void* grad_out, const void* field_in,
+                           const float* phase, int64_t phase_stride_b, const lp2d_optics* optics,
+                           void* grad_in, float* grad_phase, void* stream) {
+  int lg;
+  if (int rc = check_n(n, batch, dtype, &lg)) return rc;
+  if (!grad_out || !grad_in || !optics) return fail(LP2D_E_ARG, "null pointer");
+  if (int rc = check_optics(optics)) return rc;
+  if (grad_phase && !phase) return fail(LP2D_E_ARG, "grad_phase requires phase");
+  if ((phase || optics->lens_f != 0.0) && !field_in)
+    return fail(LP2D_E_ARG, "field_in is required with a phase screen or lens");
+  if (field_in && grad_in == field_in) return fail(LP2D_E_ARG, "grad_in must not alias field_in");
+  hipStream_t s = (hipStream_t)stream;
+  return dtype == LP2D_C64
+             ? op_adjoint<float>(n, lg, batch, grad_out, field_in, phase, phase_stride_b, optics, grad_in, grad_phase, s)
+             : op_adjoint<double>(n, lg, batch, grad_out, field_in, phase, phase_stride_b, optics, grad_in, grad_phase, s);
+}
+
+int lp2d_sum_planes(int64_t count, int batch, const float* in, float* out, void* stream) {
+  if (count < 0 || batch < 1) return fail(LP2D_E_SIZE, "bad count or batch");
+  if (!in || !out) return fail(LP2D_E_ARG, "null pointer");
+  if (count == 0) return LP2D_OK;
+  k_sum_planes<<<grid_for(count), 256, 0, (hipStream_t)stream>>>(in, out, count, batch);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? LP2D_OK : hip_fail(e, "sum kernel launch");
+}
+
+int lp2d_variant_count(int n, int dtype, int kind) {
+  ensure_registry();
+  const int lg = log2_exact(n);
+  if (lg < 1 || lg > kLogMax || (dtype != 0 && dtype != 1) || (kind != 0 && kind != 1)) return 0;
+  if (dtype == 0) return kind == 0 ? g_rf->nrows[lg] : g_rf->ncols[lg];
+  return kind == 0 ? g_rd->nrows[lg] : g_rd->ncols[lg];
+}
+
+int lp2d_set_variant(int n, int dtype, int kind, int variant) {
+  const int cnt = lp2d_variant_count(n, dtype, kind);
+  if (cnt == 0) return fail(LP2D_E_SIZE, "no variants for n=%d dtype=%d kind=%d", n, dtype, kind);
+  if (variant < 0 || variant >= cnt) return fail(LP2D_E_ARG, "variant %d out of range [0,%d)", variant, cnt);
+  g_sel[dtype][kind][log2_exact(n)] = variant;
+  return LP2D_OK;
+}
+
+int lp2d_get_variant(int n, int dtype, int kind) {
+  if (lp2d_variant_count(n, dtype, kind) == 0) return fail(LP2D_E_SIZE, "no variants");
+  return g_sel[dtype][kind][log2_exact(n)];
+}
+
+int lp2d_variant_name(int n, int dtype, int kind, int variant, char* buf, int buflen) {
+  const int cnt = lp2d_variant_count(n, dtype, kind);
+  if (variant < 0 || variant >= cnt || !buf || buflen < 1) return fail(LP2D_E_ARG, "bad variant query");
+  const int lg = log2_exact(n);
+  if (kind == 0) {
+    const int E = dtype ? g_rd->rows[lg][variant].E : g_rf->rows[lg][variant].E;
+    const int L = dtype ? g_rd->rows[lg][variant].lines : g_rf->rows[lg][variant].lines;
+    const int th = dtype ? g_rd->rows[lg][variant].threads : g_rf->rows[lg][variant].threads;
+    const int lds = dtype ? g_rd->rows[lg][variant].lds : g_rf->rows[lg][variant].lds;
+    return snprintf(buf, buflen, "rows E=%d lines=%d threads=%d lds=%d", E, L, th, lds);
+  }
+  const int E = dtype ? g_rd->cols[lg][variant].E : g_rf->cols[lg][variant].E;
+  const int W = dtype ? g_rd->cols[lg][variant].W : g_rf->cols[lg][variant].W;
+  const int th = dtype ? g_rd->cols[lg][variant].threads : g_rf->cols[lg][variant].threads;
+  const int lds = dtype ? g_rd->cols[lg][variant].lds : g_rf->cols[lg][variant].lds;
+  return snprintf(buf, buflen, "cols E=%d W=%d threads=%d lds=%d", E, W, th, lds);
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int lp2d_trace_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_trace_mu);
+  g_trace_on = on != 0;
+  return LP2D_OK;
+}
+
+int lp2d_trace_collect(lp2d_trace_stat* stats, int max_stats, int* n_stats) {
+  if (!n_stats || (max_stats > 0 && !stats)) return fail(LP2D_E_ARG, "null output");
+  std::lock_guard<std::mutex> lk(g_trace_mu);
+  std::vector<lp2d_trace_stat> agg;
+  int rc = LP2D_OK;
+  for (TraceRec& r : g_trace) {
+    float ms = 0.f;
+    hipError_t e = hipEventSynchronize(r.stop);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, r.start, r.stop);
+    if (e != hipSuccess) rc = hip_fail(e, "trace event");
+    size_t k = 0;
+    for (; k < agg.size(); ++k)
+      if (r.name == agg[k].name) break;
+    if (k == agg.size()) {
+      lp2d_trace_stat st;
+      std::memset(&st, 0, sizeof(st));
+      std::snprintf(st.name, sizeof(st.name), "%s", r.name.c_str());
+      agg.push_back(st);
+    }
+    agg[k].launches += 1;
+    agg[k].total_ms += ms;
+    g_event_pool.push_back(r.start);
+    g_event_pool.push_back(r.stop);
+  }
+  g_trace.clear();
+  *n_stats = (int)agg.size();
+  for (int i = 0; i < (int)agg.size() && i < max_stats; ++i) stats[i] = agg[i];
+  return rc;
+}
+
+// Internal (used by lp2d_batch.hip): bracket a launch with trace events.
+void* lp2d_trace_begin(void* stream, const char* name) {
+  if (!g_trace_on) return nullptr;
+  return new TraceScope((hipStream_t)stream, name);
+}
+void lp2d_trace_end(void* token) { delete (TraceScope*)token; }
+
+// Internal (used by lp2d_batch.hip): the c64 twiddle tables of the current device.
+int lp2d_twiddles_f32(const void** out) {
+  DeviceTables* t;
+  if (int rc = current_tables(&t)) return rc;
+  *out = t->twf;
+  return LP2D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,248 @@
+// Device-side FFT core for gfx950: in-register DFT butterflies and a Stockham
+// (autosort) line FFT whose inter-pass exchanges go through LDS.
+//
+// Layout convention for one line of N points processed by TL = N/E threads, E
+// elements per thread: on entry thread t holds x[t + TL*m] (m = 0..E-1), on exit it
+// holds X[t + TL*m].  Both ends are therefore stride-TL patterns, which is what makes
+// the global loads and stores of every pass coalesced and the output naturally
+// ordered.  Forward = exp(-2*pi*i*jk/N), unnormalised (numpy.fft convention).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <type_traits>
+#include <utility>
+
+namespace lp2d {
+
+template <typename T>
+struct __attribute__((aligned(2 * sizeof(T)))) cx {
+  T re, im;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> mk(T r, T i) {
+  cx<T> c;
+  c.re = r;
+  c.im = i;
+  return c;
+}
+template <typename T>
+__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return mk(a.re + b.re, a.im + b.im); }
+template <typename T>
+__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return mk(a.re - b.re, a.im - b.im); }
+// a * b
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+  return mk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+// a * conj(b)
+template <typename T>
+__device__ __forceinline__ cx<T> cmulc(cx<T> a, cx<T> b) {
+  return mk(a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im);
+}
+template <typename T>
+__device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return mk(a.re * s, a.im * s); }
+
+// compile-time loop
+template <int I, int N, typename F>
+__device__ __forceinline__ void sfor_impl(F&& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    sfor_impl<I + 1, N>(f);
+  }
+}
+template <int N, typename F>
+__device__ __forceinline__ void sfor(F&& f) { sfor_impl<0, N>(f); }
+
+// cos(2*pi*k/64), k = 0..16 (first quadrant); the rest follows by symmetry.
+__host__ __device__ constexpr double kQuarterCos64(int k) {
+  constexpr double q[17] = {1.0,
+                            0.9951847266721969,
+                            0.9807852804032304,
+                            0.9569403357322088,
+                            0.9238795325112867,
+                            0.881921264348355,
+                            0.8314696123025452,
+                            0.773010453362737,
+                            0.7071067811865476,
+                            0.6343932841636455,
+                            0.5555702330196023,
+                            0.4713967368259978,
+                            0.38268343236508984,
+                            0.29028467725446233,
+                            0.19509032201612833,
+                            0.09801714032956077,
+                            0.0};
+  return q[k];
+}
+__host__ __device__ constexpr double kCos64(int k) {
+  k &= 63;
+  return k <= 16 ? kQuarterCos64(k) : k <= 32 ? -kQuarterCos64(32 - k)
+                                   : k <= 48 ? -kQuarterCos64(k - 32)
+                                             : kQuarterCos64(64 - k);
+}
+__host__ __device__ constexpr double kSin64(int k) { return kCos64(k - 16); }
+
+// a * exp(-+2*pi*i*E/R) with E a compile-time exponent (forward: minus sign).
+template <typename T, int R, int E, bool INV>
+__device__ __forceinline__ cx<T> twc(cx<T> a) {
+  static_assert(64 % R == 0, "radix must divide 64");
+  constexpr int k = ((E % R) + R) % R * (64 / R);
+  if constexpr (k == 0) {
+    return a;
+  } else if constexpr (k == 16) {  // fwd: -i, inv: +i
+    return INV ? mk(-a.im, a.re) : mk(a.im, -a.re);
+  } else if constexpr (k == 32) {
+    return mk(-a.re, -a.im);
+  } else if constexpr (k == 48) {  // fwd: +i, inv: -i
+    return INV ? mk(a.im, -a.re) : mk(-a.im, a.re);
+  } else {
+    constexpr double c = kCos64(k);
+    constexpr double s = INV ? kSin64(k) : -kSin64(k);
+    return cmul(a, mk((T)c, (T)s));
+  }
+}
+
+// In-register DFT of length R (power of two <= 64), natural order in and out.
+// Cooley-Tukey decimation in time, R = R1*R2 with R1 = 4 (or 2).
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dft(cx<T> (&x)[R]) {
+  if constexpr (R == 1) {
+  } else if constexpr (R == 2) {
+    cx<T> a = x[0], b = x[1];
+    x[0] = a + b;
+    x[1] = a - b;
+  } else if constexpr (R == 4) {
+    cx<T> t0 = x[0] + x[2], t1 = x[0] - x[2], t2 = x[1] + x[3];
+    cx<T> t3 = twc<T, 4, 1, INV>(x[1] - x[3]);
+    x[0] = t0 + t2;
+    x[2] = t0 - t2;
+    x[1] = t1 + t3;
+    x[3] = t1 - t3;
+  } else {
+    constexpr int R1 = (R == 8) ? 2 : 4;
+    constexpr int R2 = R / R1;
+    // step 1: length-R2 DFTs over the stride-R1 subsequences, then twiddle
+    sfor<R1>([&](auto n1c) {
+      constexpr int n1 = decltype(n1c)::value;
+      cx<T> s[R2];
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) s[n2] = x[n2 * R1 + n1];
+      dft<T, R2, INV>(s);
+      sfor<R2>([&](auto k2c) {
+        constexpr int k2 = decltype(k2c)::value;
+        x[k2 * R1 + n1] = twc<T, R, n1 * k2, INV>(s[k2]);
+      });
+    });
+    // step 2: length-R1 DFTs across, output X[k1*R2 + k2]
+    cx<T> out[R];
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) {
+      cx<T> s[R1];
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) s[n1] = x[k2 * R1 + n1];
+      dft<T, R1, INV>(s);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) out[k1 * R2 + k2] = s[k1];
+    }
+#pragma unroll
+    for (int i = 0; i < R; ++i) x[i] = out[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LDS accessors.  `pad(i)` inserts one element every 2^PS elements so that the
+// stride-R writes of the early Stockham passes spread over the banks.
+template <int PS>
+__device__ __forceinline__ int pad_idx(int i) { return i + (i >> PS); }
+
+template <int N, int PS>
+struct PaddedLen {
+  static constexpr int value = N + (N >> PS);
+};
+
+// One line stored contiguously (rows).
+template <typename T, int PS>
+struct LineAcc {
+  cx<T>* base;
+  __device__ __forceinline__ void st(int i, cx<T> v) const { base[pad_idx<PS>(i)] = v; }
+  __device__ __forceinline__ cx<T> ld(int i) const { return base[pad_idx<PS>(i)]; }
+};
+
+// W lines interleaved element by element (column strips): element i of line c sits
+// at pad(i)*W + c, so lanes that differ in c touch consecutive addresses.
+template <typename T, int PS, int W>
+struct InterAcc {
+  cx<T>* base;  // already offset by the line index c
+  __device__ __forceinline__ void st(int i, cx<T> v) const { base[pad_idx<PS>(i) * W] = v; }
+  __device__ __forceinline__ cx<T> ld(int i) const { return base[pad_idx<PS>(i) * W]; }
+};
+
+// Strided access inside a 2-D LDS tile (column c of a row-major tile with pitch P).
+template <typename T, int P>
+struct StrideAcc {
+  cx<T>* base;  // offset to column c (already padded)
+  __device__ __forceinline__ void st(int i, cx<T> v) const { base[i * P] = v; }
+  __device__ __forceinline__ cx<T> ld(int i) const { return base[i * P]; }
+};
+
+// ---------------------------------------------------------------------------
+// Stockham autosort passes.  Pass with stride NS (product of the previous radices)
+// and radix R: butterfly j in [0, N/R) reads in[j + r*N/R], multiplies element r by
+// w_{NS*R}^{(j mod NS)*r}, runs DFT_R and writes out[(j/NS)*NS*R + (j mod NS) + r*NS].
+// Thread t owns the butterflies j = t + TL*q (q < E/R), whose inputs are exactly the
+// registers v[q + r*E/R].  The last pass writes back into registers in the stride-TL
+// layout, so only the passes before it exchange through LDS.
+//
+// tw points at the twiddle table of length N: tw[i] = exp(-2*pi*i*i/N).
+template <typename T, int N, int E, bool INV, int NS, typename Acc>
+__device__ __forceinline__ void stockham(cx<T> (&v)[E], int t, const Acc& acc,
+                                         const cx<T>* __restrict__ tw) {
+  constexpr int REM = N / NS;
+  constexpr int R = REM >= E ? E : REM;
+  constexpr int B = E / R;
+  constexpr int TL = N / E;
+  constexpr bool LAST = (NS * R == N);
+  static_assert(E % R == 0, "radix must divide the per-thread element count");
+#pragma unroll
+  for (int q = 0; q < B; ++q) {
+    const int j = t + TL * q;
+    const int k = j & (NS - 1);
+    cx<T> a[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) a[r] = v[q + r * B];
+    if constexpr (NS > 1) {
+      constexpr int TS = N / (NS * R);
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const cx<T> w = tw[k * r * TS];
+        a[r] = INV ? cmulc(a[r], w) : cmul(a[r], w);
+      }
+    }
+    dft<T, R, INV>(a);
+    if constexpr (LAST) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q + r * B] = a[r];
+    } else {
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc.st(base + r * NS, a[r]);
+    }
+  }
+  if constexpr (!LAST) {
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = acc.ld(t + TL * m);
+    __syncthreads();
+    stockham<T, N, E, INV, NS * R>(v, t, acc, tw);
+  }
+}
+
+template <typename T, int N, int E, bool INV, typename Acc>
+__device__ __forceinline__ void line_fft(cx<T> (&v)[E], int t, const Acc& acc,
+                                         const cx<T>* __restrict__ tw) {
+  static_assert((N & (N - 1)) == 0 && (E & (E - 1)) == 0 && E <= N, "powers of two");
+  stockham<T, N, E, INV, 1>(v, t, acc, tw);
+}
+
+}  // namespace lp2d

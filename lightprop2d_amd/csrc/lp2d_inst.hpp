@@ -1,0 +1,160 @@
+// Kernel templates, their launchers and the per-size variant lists.  Included by the
+// per-precision translation units (lp2d_f32_*.hip, lp2d_f64_*.hip) so the large set
+// of instantiations compiles in parallel.
+#pragma once
+#include "lp2d_registry.hpp"
+
+namespace lp2d {
+
+constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n >> 1); }
+constexpr int cmin(int a, int b) { return a < b ? a : b; }
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int floor_pow2(int n) { return n < 2 ? 1 : 2 * floor_pow2(n >> 1); }
+constexpr int kLdsBudget = 160 * 1024 - 2048;
+
+// ----------------------------------------------------------------- kernels
+template <typename T, int N, int E, int LINES, bool INV, int PS>
+__global__ __launch_bounds__((N / E) * LINES) void k_rows(PassArgs<T> a) {
+  __shared__ cx<T> lds[LINES * PaddedLen<N, PS>::value];
+  constexpr int TPF = (N + LINES - 1) / LINES;  // row tiles per field
+  const int tile = blockIdx.x;
+  const int b = tile / TPF;
+  const int y0 = (tile - b * TPF) * LINES;
+  rows_body<T, N, E, LINES, INV, PS>(a, b, y0, cmin(LINES, N - y0), lds, threadIdx.x);
+}
+
+template <typename T, int N, int E, int W, int MODE, int PS>
+__global__ __launch_bounds__((N / E) * W) void k_cols(PassArgs<T> a) {
+  __shared__ cx<T> lds[W * PaddedLen<N, PS>::value];
+  constexpr int SPF = N / W;  // column strips per field
+  const int g = gridDim.x, i = blockIdx.x;
+  // Workgroups go to the 8 XCDs round robin.  Give each XCD a contiguous run of
+  // strips so that narrow strips sharing 128-byte lines meet in the same L2.
+  const int blk = (g % 8 == 0) ? (i % 8) * (g / 8) + i / 8 : i;
+  const int b = blk / SPF;
+  const int x0 = (blk - b * SPF) * W;
+  cols_body<T, N, E, W, MODE, PS>(a, b, x0, lds, threadIdx.x);
+}
+
+template <typename T, int N, int E, int PS>
+__global__ __launch_bounds__(N * (N / E)) void k_small(PassArgs<T> a, int mode) {
+  __shared__ cx<T> lds[N * SmallPitch<N, PS>::value];
+  small_body<T, N, E, PS>(a, blockIdx.x, mode, lds, threadIdx.x);
+}
+
+// ----------------------------------------------------------------- shapes
+template <typename T, int N, int E>
+struct RowShape {
+  static constexpr int TL = N / E;
+  static constexpr int PS = cmax(1, ilog2c(E));
+  static constexpr int LINES = cmax(1, cmin(N, 256 / TL));
+  static constexpr int THREADS = TL * LINES;
+  static constexpr int LDS = LINES * PaddedLen<N, PS>::value * (int)sizeof(cx<T>);
+};
+
+template <typename T, int N, int E, int WT>
+struct ColShape {
+  static constexpr int TL = N / E;
+  static constexpr int PS = cmax(1, ilog2c(E));
+  static constexpr int L = PaddedLen<N, PS>::value;
+  static constexpr int W = floor_pow2(
+      cmax(1, cmin(cmin(WT, N), cmin(1024 / TL, kLdsBudget / (L * (int)sizeof(cx<T>))))));
+  static constexpr int THREADS = TL * W;
+  static constexpr int LDS = W * L * (int)sizeof(cx<T>);
+};
+
+template <typename T, int N, int EMAX>
+struct SmallShape {
+  static constexpr int E = cmin(N, EMAX);
+  static constexpr int PS = cmax(1, ilog2c(E));
+  static constexpr int THREADS = N * (N / E);
+  static constexpr int LDS = N * SmallPitch<N, PS>::value * (int)sizeof(cx<T>);
+  static constexpr bool OK = THREADS <= 1024 && LDS <= kLdsBudget;
+};
+
+// ----------------------------------------------------------------- launchers
+template <typename T, int N, int E, int LINES, int PS>
+hipError_t launch_rows(const PassArgs<T>& a, bool inv, hipStream_t s) {
+  constexpr int threads = (N / E) * LINES;
+  constexpr int TPF = (N + LINES - 1) / LINES;
+  const unsigned grid = (unsigned)((long long)a.batch * TPF);
+  if (inv)
+    k_rows<T, N, E, LINES, true, PS><<<grid, threads, 0, s>>>(a);
+  else
+    k_rows<T, N, E, LINES, false, PS><<<grid, threads, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+template <typename T, int N, int E, int W, int PS>
+hipError_t launch_cols(const PassArgs<T>& a, int mode, hipStream_t s) {
+  constexpr int threads = (N / E) * W;
+  const unsigned grid = (unsigned)((long long)a.batch * (N / W));
+  switch (mode) {
+    case MODE_FWD:
+      k_cols<T, N, E, W, MODE_FWD, PS><<<grid, threads, 0, s>>>(a);
+      break;
+    case MODE_INV:
+      k_cols<T, N, E, W, MODE_INV, PS><<<grid, threads, 0, s>>>(a);
+      break;
+    default:
+      k_cols<T, N, E, W, MODE_FWD | MODE_INV, PS><<<grid, threads, 0, s>>>(a);
+      break;
+  }
+  return hipGetLastError();
+}
+
+template <typename T, int N, int E, int PS>
+hipError_t launch_small(const PassArgs<T>& a, int mode, hipStream_t s) {
+  k_small<T, N, E, PS><<<(unsigned)a.batch, N * (N / E), 0, s>>>(a, mode);
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------------------- registration
+template <typename T, int N, int E>
+void add_row(Registry<T>& r) {
+  using S = RowShape<T, N, E>;
+  constexpr int lg = ilog2c(N);
+  int& k = r.nrows[lg];
+  if (k >= kMaxVariants) return;
+  r.rows[lg][k++] = RowVar<T>{&launch_rows<T, N, E, S::LINES, S::PS>, E, S::LINES, S::THREADS,
+                              S::LDS};
+}
+
+template <typename T, int N, int E, int WT>
+void add_col(Registry<T>& r) {
+  using S = ColShape<T, N, E, WT>;
+  constexpr int lg = ilog2c(N);
+  int& k = r.ncols[lg];
+  if (k >= kMaxVariants) return;
+  r.cols[lg][k++] = ColVar<T>{&launch_cols<T, N, E, S::W, S::PS>, E, S::W, S::THREADS, S::LDS};
+}
+
+template <typename T, int N>
+void register_n(Registry<T>& r) {
+  constexpr int lg = ilog2c(N);
+  constexpr bool F32 = sizeof(T) == 4;
+  constexpr int EMAX = F32 ? 16 : 8;
+  constexpr int E0 = cmin(N, EMAX);
+  constexpr int E1 = cmin(N, EMAX / 2);
+  r.nrows[lg] = 0;
+  r.ncols[lg] = 0;
+  add_row<T, N, E0>(r);
+  add_col<T, N, E0, F32 ? 16 : 8>(r);
+  if constexpr (E1 != E0 && N >= 16) {
+    add_row<T, N, E1>(r);
+    add_col<T, N, E1, F32 ? 16 : 8>(r);
+  }
+  if constexpr (F32 && N >= 256) {
+    add_row<T, N, 32>(r);
+    add_col<T, N, 32, 16>(r);
+  }
+  using SS = SmallShape<T, N, EMAX>;
+  if constexpr (SS::OK) {
+    r.small[lg] = SmallVar<T>{&launch_small<T, N, SS::E, SS::PS>, SS::E, SS::THREADS, SS::LDS};
+    r.has_small[lg] = true;
+  } else {
+    r.has_small[lg] = false;
+  }
+}
+
+}  // namespace lp2d

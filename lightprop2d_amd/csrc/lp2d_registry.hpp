@@ -1,0 +1,52 @@
+// Kernel-variant registry: for every (precision, n) the launchable schedules of the
+// row pass, the column pass and the whole-field-in-LDS kernel.  The autotuner picks
+// one variant per (n, precision, kind) through lp2d_set_variant.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "lp2d_kernels.hpp"
+
+namespace lp2d {
+
+template <typename T>
+using RowLaunch = hipError_t (*)(const PassArgs<T>&, bool inv, hipStream_t);
+template <typename T>
+using ColLaunch = hipError_t (*)(const PassArgs<T>&, int mode, hipStream_t);
+template <typename T>
+using SmallLaunch = hipError_t (*)(const PassArgs<T>&, int mode, hipStream_t);
+
+template <typename T>
+struct RowVar {
+  RowLaunch<T> fn;
+  int E, lines, threads, lds;
+};
+template <typename T>
+struct ColVar {
+  ColLaunch<T> fn;
+  int E, W, threads, lds;
+};
+template <typename T>
+struct SmallVar {
+  SmallLaunch<T> fn;
+  int E, threads, lds;
+};
+
+constexpr int kMaxVariants = 4;
+constexpr int kLogMax = 13;  // n <= 8192
+
+template <typename T>
+struct Registry {
+  RowVar<T> rows[kLogMax + 1][kMaxVariants];
+  int nrows[kLogMax + 1];
+  ColVar<T> cols[kLogMax + 1][kMaxVariants];
+  int ncols[kLogMax + 1];
+  SmallVar<T> small[kLogMax + 1];
+  bool has_small[kLogMax + 1];
+};
+
+void register_f32_a(Registry<float>& r);   // n = 2 .. 256
+void register_f32_b(Registry<float>& r);   // n = 512 .. 8192
+void register_f64_a(Registry<double>& r);  // n = 2 .. 256
+void register_f64_b(Registry<double>& r);  // n = 512 .. 8192
+
+}  // namespace lp2d

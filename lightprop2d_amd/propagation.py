@@ -1,0 +1,336 @@
+"""Tensor-level operations on PyTorch-ROCm device tensors, each one a call into liblp2d.
+
+Fields are complex64 / complex128 tensors of shape (n, n) or (batch, n, n), rows = y,
+as in the reference (lightprop2d/beam2d.py:299-308).  Every function runs on the
+caller's current HIP stream; none of them has a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Iterable, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from ._lib import Lp2dError
+
+__all__ = ["fft2", "ifft2", "kz_grid", "spatial_apply", "spectral_apply", "propagate",
+           "propagate_adjoint", "intensity", "sum_planes", "UnevenBatch", "optics",
+           "Lp2dError"]
+
+_DT = {torch.complex64: L.LP2D_C64, torch.complex128: L.LP2D_C128}
+_MASK_DT = {torch.bool: L.LP2D_MASK_U8, torch.uint8: L.LP2D_MASK_U8,
+            torch.float32: L.LP2D_MASK_F32, torch.float64: L.LP2D_MASK_F64,
+            torch.complex64: L.LP2D_MASK_C64, torch.complex128: L.LP2D_MASK_C128}
+
+
+def _stream(t: torch.Tensor):
+    return C.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _field_geom(t: torch.Tensor, name="field"):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if t.device.type != "cuda":
+        raise Lp2dError(f"{name} must live on a HIP device (got {t.device})")
+    if t.dtype not in _DT:
+        raise Lp2dError(f"{name} must be complex64 or complex128 (got {t.dtype})")
+    if t.dim() not in (2, 3) or t.shape[-1] != t.shape[-2]:
+        raise Lp2dError(f"{name} must have shape (n, n) or (batch, n, n), got {tuple(t.shape)}")
+    if not t.is_contiguous():
+        raise Lp2dError(f"{name} must be contiguous")
+    n = t.shape[-1]
+    batch = t.shape[0] if t.dim() == 3 else 1
+    return n, batch, _DT[t.dtype]
+
+
+def optics(area_size: float, wl: float, z: float = 0.0, aperture_radius: float = 0.0,
+           lens_f: float = 0.0, evanescent: str = "reference") -> L.lp2d_optics:
+    if evanescent not in L.EVANESCENT:
+        raise ValueError(f"evanescent must be one of {sorted(L.EVANESCENT)}")
+    return L.lp2d_optics(float(area_size), float(wl), float(z),
+                         float(aperture_radius or 0.0), float(lens_f or 0.0),
+                         L.EVANESCENT[evanescent], 0)
+
+
+def _as_device_tensor(m, device):
+    if isinstance(m, torch.Tensor):
+        return m.to(device)
+    if isinstance(m, np.ndarray):
+        return torch.from_numpy(np.ascontiguousarray(m)).to(device)
+    if np.isscalar(m):
+        return torch.tensor(m, device=device)
+    raise TypeError(f"unsupported multiplier type {type(m)}")
+
+
+def _masks(masks: Sequence, n: int, batch: int, device):
+    """Build lp2d_mask descriptors (broadcast strides, no copies when possible).
+    Returns (array, keepalive list)."""
+    keep = []
+    arr = (L.lp2d_mask * max(1, len(masks)))()
+    for i, m in enumerate(masks):
+        t = _as_device_tensor(m, device)
+        if t.dtype not in _MASK_DT:
+            t = t.to(torch.float64)
+        if t.dim() > 3:
+            raise Lp2dError(f"mask {i} has {t.dim()} dims; at most (batch, n, n)")
+        try:
+            e = torch.broadcast_to(t, (batch, n, n))
+        except RuntimeError as exc:
+            raise Lp2dError(f"mask {i} of shape {tuple(t.shape)} does not broadcast to "
+                            f"({batch}, {n}, {n})") from exc
+        keep.append(t)
+        sb, sy, sx = e.stride()
+        arr[i] = L.lp2d_mask(e.data_ptr(), _MASK_DT[t.dtype], 0, sb, sy, sx)
+    return arr, keep
+
+
+def kz_grid(n: int, area_size: float, wl: float, device=None) -> torch.Tensor:
+    """Kz = nan_to_num(sqrt(k0^2 - Kx^2 - Ky^2)), FP64 (n, n) (beam2d.py:304-308)."""
+    lib = L.lib()
+    dev = torch.device(device) if device is not None else torch.device("cuda")
+    out = torch.empty((n, n), dtype=torch.float64, device=dev)
+    o = optics(area_size, wl)
+    with torch.cuda.device(out.device):
+        L.check(lib.lp2d_kz_grid(n, C.byref(o), _ptr(out), _stream(out)), "kz_grid")
+    return out
+
+
+def fft2(field: torch.Tensor, out: Optional[torch.Tensor] = None, inverse: bool = False):
+    """2-D FFT over the last two axes, numpy normalisation (beam2d.py:207-229)."""
+    lib = L.lib()
+    n, batch, dt = _field_geom(field)
+    if out is None:
+        out = torch.empty_like(field)
+    else:
+        _field_geom(out, "out")
+    with torch.cuda.device(field.device):
+        L.check(lib.lp2d_fft2(n, batch, dt, _ptr(field), _ptr(out), int(bool(inverse)),
+                              _stream(field)), "fft2")
+    return out
+
+
+def ifft2(spectrum: torch.Tensor, out: Optional[torch.Tensor] = None):
+    return fft2(spectrum, out, inverse=True)
+
+
+def spatial_apply(field: torch.Tensor, masks: Sequence = (), lens_f: float = 0.0,
+                  area_size: float = 0.0, wl: float = 0.0, phase: Optional[torch.Tensor] = None,
+                  spectrum_out: Optional[torch.Tensor] = None):
+    """field *= masks * lens * exp(i*phase) in place; spectrum_out = fft2(field)."""
+    lib = L.lib()
+    n, batch, dt = _field_geom(field)
+    if len(masks) > 2:
+        raise Lp2dError("at most two masks per call")
+    marr, keep = _masks(masks, n, batch, field.device)
+    o = optics(area_size, wl, lens_f=lens_f) if lens_f else None
+    ph, psb = _phase(phase, n, batch, field.device)
+    if spectrum_out is not None:
+        _field_geom(spectrum_out, "spectrum_out")
+    with torch.cuda.device(field.device):
+        L.check(lib.lp2d_spatial_apply(n, batch, dt, _ptr(field), marr, len(masks),
+                                       C.byref(o) if o is not None else None, _ptr(ph), psb,
+                                       _ptr(spectrum_out), _stream(field)), "spatial_apply")
+    del keep
+    return spectrum_out
+
+
+def spectral_apply(spectrum: torch.Tensor, masks: Sequence = (), z: float = 0.0,
+                   area_size: float = 0.0, wl: float = 0.0, aperture_radius: float = 0.0,
+                   evanescent: str = "reference", field_out: Optional[torch.Tensor] = None):
+    """spectrum *= H(z) * aperture * masks in place; field_out = ifft2(spectrum)."""
+    lib = L.lib()
+    n, batch, dt = _field_geom(spectrum, "spectrum")
+    if len(masks) > 2:
+        raise Lp2dError("at most two masks per call")
+    marr, keep = _masks(masks, n, batch, spectrum.device)
+    o = optics(area_size, wl, z=z, aperture_radius=aperture_radius,
+               evanescent=evanescent) if (z or aperture_radius) else None
+    if field_out is not None:
+        _field_geom(field_out, "field_out")
+    with torch.cuda.device(spectrum.device):
+        L.check(lib.lp2d_spectral_apply(n, batch, dt, _ptr(spectrum), marr, len(masks),
+                                        C.byref(o) if o is not None else None,
+                                        _ptr(field_out), _stream(spectrum)), "spectral_apply")
+    del keep
+    return field_out
+
+
+def _phase(phase, n, batch, device):
+    if phase is None:
+        return None, 0
+    if not isinstance(phase, torch.Tensor):
+        phase = torch.as_tensor(np.asarray(phase), device=device)
+    if phase.dtype != torch.float32 or phase.device != device or not phase.is_contiguous():
+        raise Lp2dError("phase must be a contiguous float32 tensor on the field's device")
+    if phase.shape == (n, n):
+        return phase, 0
+    if phase.shape == (batch, n, n):
+        return phase, n * n
+    raise Lp2dError(f"phase must have shape ({n}, {n}) or ({batch}, {n}, {n})")
+
+
+def propagate(field: torch.Tensor, z: float, area_size: float, wl: float,
+              phase: Optional[torch.Tensor] = None, aperture_radius: float = 0.0,
+              lens_f: float = 0.0, evanescent: str = "reference",
+              out: Optional[torch.Tensor] = None, spectrum_out: Optional[torch.Tensor] = None):
+    """Field-to-field angular-spectrum propagation (one fused call):
+
+        out = ifft2(H(z) * aperture * fft2(field * exp(i*phase) * lens(lens_f)))
+
+    H(z) = exp(-2*pi*i*frac(Kz*z)) with Kz*z in FP64 (beam2d.py:485-489), evanescent
+    modes pass unchanged unless `evanescent` is "zero" or "decay".  `out` may be
+    `field` (in place).  Not differentiable; see lightprop2d_amd.autograd.
+    """
+    lib = L.lib()
+    n, batch, dt = _field_geom(field)
+    if out is None:
+        out = torch.empty_like(field)
+    else:
+        _field_geom(out, "out")
+    ph, psb = _phase(phase, n, batch, field.device)
+    o = optics(area_size, wl, z=z, aperture_radius=aperture_radius, lens_f=lens_f,
+               evanescent=evanescent)
+    with torch.cuda.device(field.device):
+        L.check(lib.lp2d_propagate(n, batch, dt, _ptr(field), _ptr(out), _ptr(ph), psb,
+                                   C.byref(o), _ptr(spectrum_out), _stream(field)), "propagate")
+    return out
+
+
+def propagate_adjoint(grad_out: torch.Tensor, field_in: Optional[torch.Tensor], z: float,
+                      area_size: float, wl: float, phase: Optional[torch.Tensor] = None,
+                      aperture_radius: float = 0.0, lens_f: float = 0.0,
+                      evanescent: str = "reference", want_phase_grad: bool = True,
+                      out: Optional[torch.Tensor] = None):
+    """Adjoint of `propagate` (vector-Jacobian product, PyTorch complex convention).
+
+    Returns (grad_field, grad_phase) where grad_phase has the phase's shape (summed
+    over the batch for a shared (n, n) screen) or is None."""
+    lib = L.lib()
+    n, batch, dt = _field_geom(grad_out, "grad_out")
+    if out is None:
+        out = torch.empty_like(grad_out)
+    ph, psb = _phase(phase, n, batch, grad_out.device)
+    gph = None
+    if ph is not None and want_phase_grad:
+        gph = torch.empty((batch, n, n), dtype=torch.float32, device=grad_out.device)
+    if field_in is not None:
+        _field_geom(field_in, "field_in")
+    o = optics(area_size, wl, z=z, aperture_radius=aperture_radius, lens_f=lens_f,
+               evanescent=evanescent)
+    with torch.cuda.device(grad_out.device):
+        L.check(lib.lp2d_propagate_adjoint(n, batch, dt, _ptr(grad_out), _ptr(field_in),
+                                           _ptr(ph), psb, C.byref(o), _ptr(out), _ptr(gph),
+                                           _stream(grad_out)), "propagate_adjoint")
+        if gph is not None:
+            if psb == 0:
+                if batch > 1:
+                    red = torch.empty((n, n), dtype=torch.float32, device=grad_out.device)
+                    sum_planes(gph, red)
+                    gph = red
+                else:
+                    gph = gph.view(n, n)
+    return out, gph
+
+
+def sum_planes(planes: torch.Tensor, out: torch.Tensor):
+    """out = planes.sum(0) for a float32 (batch, ...) tensor, in a HIP kernel."""
+    lib = L.lib()
+    b = planes.shape[0]
+    cnt = planes[0].numel()
+    L.check(lib.lp2d_sum_planes(cnt, b, _ptr(planes), _ptr(out), _stream(planes)), "sum_planes")
+    return out
+
+
+def intensity(field: torch.Tensor, out: Optional[torch.Tensor] = None):
+    """|field|^2 (beam2d.py:717): float32 for complex64, float64 for complex128."""
+    lib = L.lib()
+    if field.dtype not in _DT or field.device.type != "cuda" or not field.is_contiguous():
+        raise Lp2dError("intensity needs a contiguous complex tensor on a HIP device")
+    rdt = torch.float32 if field.dtype == torch.complex64 else torch.float64
+    if out is None:
+        out = torch.empty(field.shape, dtype=rdt, device=field.device)
+    with torch.cuda.device(field.device):
+        L.check(lib.lp2d_intensity(field.numel(), _DT[field.dtype], _ptr(field), _ptr(out),
+                                   _stream(field)), "intensity")
+    return out
+
+
+class UnevenBatch:
+    """Many independent complex64 fields of different sizes (64..4096), distances and
+    optics, propagated by three persistent launches per call (whole-field tiles for
+    n <= 128, row/column tiles otherwise).
+
+    Fields are propagated src -> dst (dst may be src).  `run()` allocates nothing and
+    can be captured in a CUDA/HIP graph (`capture_graph()`).
+    """
+
+    def __init__(self, srcs: Sequence[torch.Tensor], dsts: Sequence[torch.Tensor],
+                 zs: Iterable[float], area_size, wl, aperture_radius=0.0):
+        lib = L.lib()
+        srcs = list(srcs)
+        dsts = list(dsts)
+        zs = list(zs)
+        m = len(srcs)
+        if not (len(dsts) == m == len(zs)) or m == 0:
+            raise Lp2dError("srcs, dsts and zs must have the same non-zero length")
+        areas = _per_field(area_size, m)
+        wls = _per_field(wl, m)
+        aps = _per_field(aperture_radius, m)
+        descs = (L.lp2d_field_desc * m)()
+        for i, (s, d) in enumerate(zip(srcs, dsts)):
+            n, b, dt = _field_geom(s, f"srcs[{i}]")
+            if b != 1 or s.dim() != 2 or dt != L.LP2D_C64:
+                raise Lp2dError("uneven batches take complex64 (n, n) fields")
+            if d.shape != s.shape or d.dtype != s.dtype or not d.is_contiguous():
+                raise Lp2dError(f"dsts[{i}] must match srcs[{i}]")
+            descs[i] = L.lp2d_field_desc(n, 0, s.data_ptr(), d.data_ptr(), float(areas[i]),
+                                         float(wls[i]), float(zs[i]), float(aps[i]))
+        self._keep = (srcs, dsts)
+        self.device = srcs[0].device
+        self._h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            L.check(lib.lp2d_batch_create(descs, m, L.LP2D_C64, C.byref(self._h)), "batch_create")
+        self.count = m
+        self.elements = sum(s.numel() for s in srcs)
+        self._graph = None
+
+    def set_grid(self, workgroups: int):
+        L.check(L.lib().lp2d_batch_set_grid(self._h, int(workgroups)), "batch_set_grid")
+
+    def run(self):
+        with torch.cuda.device(self.device):
+            L.check(L.lib().lp2d_batch_run(self._h, C.c_void_p(
+                torch.cuda.current_stream(self.device).cuda_stream)), "batch_run")
+
+    def capture_graph(self):
+        """Capture run() into a torch.cuda.CUDAGraph (HIP graph) and return it."""
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream(self.device)
+        s.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                self.run()
+        torch.cuda.current_stream(self.device).wait_stream(s)
+        self._graph = g
+        return g
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and L._lib is not None:
+            L._lib.lp2d_batch_destroy(h)
+            self._h = None
+
+
+def _per_field(v, m):
+    if np.isscalar(v):
+        return [float(v)] * m
+    v = list(v)
+    if len(v) != m:
+        raise Lp2dError("per-field parameter list has the wrong length")
+    return v

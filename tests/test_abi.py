@@ -1,0 +1,88 @@
+"""CPU-side checks of the C ABI: the library loads, exports every symbol that
+include/lp2d.h declares, and rejects bad arguments before touching a device."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "lp2d.h")
+
+
+def header_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(lp2d_[a-z0-9_]+)\s*\(", src)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from lightprop2d_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("liblp2d.so not built")
+    return _lib.load()
+
+
+def test_header_declares_the_bound_symbols():
+    from lightprop2d_amd import _lib
+    assert header_functions() == sorted(_lib.EXPORTED)
+
+
+def test_every_declared_symbol_is_exported(lib):
+    for name in header_functions():
+        assert hasattr(lib, name), name
+
+
+def test_host_only_queries(lib):
+    assert lib.lp2d_version() >= 10000
+    assert lib.lp2d_max_n(0) == 8192 and lib.lp2d_max_n(1) == 8192 and lib.lp2d_max_n(7) == 0
+    buf = C.create_string_buffer(128)
+    for n in [2, 16, 128, 1024, 8192]:
+        for dt in (0, 1):
+            for kind in (0, 1):
+                cnt = lib.lp2d_variant_count(n, dt, kind)
+                assert cnt >= 1
+                for v in range(cnt):
+                    assert lib.lp2d_variant_name(n, dt, kind, v, buf, 128) > 0
+    assert lib.lp2d_variant_count(3, 0, 0) == 0
+    assert lib.lp2d_set_variant(1024, 0, 1, 99) == -1
+
+
+def test_argument_errors_before_any_device_work(lib):
+    from lightprop2d_amd import _lib
+    p = C.c_void_p(16)
+    assert lib.lp2d_fft2(3, 1, 0, p, p, 0, None) == _lib.LP2D_E_SIZE
+    assert b"power of two" in lib.lp2d_last_error()
+    assert lib.lp2d_fft2(16384, 1, 0, p, p, 0, None) == _lib.LP2D_E_SIZE
+    assert lib.lp2d_fft2(64, 1, 9, p, p, 0, None) == _lib.LP2D_E_DTYPE
+    assert lib.lp2d_fft2(64, 0, 0, p, p, 0, None) == _lib.LP2D_E_SIZE
+    assert lib.lp2d_fft2(64, 1, 0, None, p, 0, None) == _lib.LP2D_E_ARG
+    o = _lib.lp2d_optics(1.0, 1e-4, 0.1, 0.0, 0.0, 0, 0)
+    assert lib.lp2d_propagate(64, 1, 0, p, p, None, 0, None, None, None) == _lib.LP2D_E_ARG
+    bad = _lib.lp2d_optics(-1.0, 1e-4, 0.1, 0.0, 0.0, 0, 0)
+    assert lib.lp2d_propagate(64, 1, 0, p, p, None, 0, C.byref(bad), None, None) == _lib.LP2D_E_ARG
+    assert lib.lp2d_propagate(64, 1, 0, p, p, None, 0, C.byref(o), p, None) == _lib.LP2D_E_ARG
+    assert lib.lp2d_propagate_adjoint(64, 1, 0, p, None, p, 0, C.byref(o), p, None,
+                                      None) == _lib.LP2D_E_ARG
+    m = (_lib.lp2d_mask * 1)(_lib.lp2d_mask(16, 42, 0, 0, 0, 0))
+    assert lib.lp2d_spatial_apply(64, 1, 0, p, m, 1, None, None, 0, None, None) == _lib.LP2D_E_DTYPE
+    assert lib.lp2d_spectral_apply(64, 1, 0, p, m, 3, None, None, None) == _lib.LP2D_E_ARG
+    assert lib.lp2d_intensity(-1, 0, p, p, None) == _lib.LP2D_E_SIZE
+    d = (_lib.lp2d_field_desc * 1)(_lib.lp2d_field_desc(48, 0, 16, 16, 1.0, 1e-4, 0.1, 0.0))
+    h = C.c_void_p()
+    assert lib.lp2d_batch_create(d, 1, 0, C.byref(h)) == _lib.LP2D_E_SIZE
+    assert lib.lp2d_batch_create(d, 1, 1, C.byref(h)) == _lib.LP2D_E_DTYPE
+
+
+def test_python_layer_rejects_cpu_tensors():
+    import torch
+    from lightprop2d_amd import propagation as P
+    from lightprop2d_amd._lib import Lp2dError, LIB_PATH
+    if not os.path.exists(LIB_PATH):
+        pytest.skip("liblp2d.so not built")
+    with pytest.raises(Lp2dError):
+        P.fft2(torch.zeros((8, 8), dtype=torch.complex64))
+    with pytest.raises(Lp2dError):
+        P.propagate(torch.zeros((8, 8), dtype=torch.complex64), 1e-3, 1e-2, 5e-5)

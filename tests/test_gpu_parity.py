@@ -1,0 +1,360 @@
+"""GPU parity: the HIP path against the oracle on the same seeded inputs.
+
+Tolerances (relative L2 over the whole array):
+  complex64  results vs the complex128 oracle:  1e-5  (the stated target)
+  complex128 results vs the complex128 oracle:  1e-10
+All calls go through liblp2d (C ABI) via lightprop2d_amd.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel_l2
+import oracle
+from oracle import profiles as OP
+from oracle.angular_spectrum import OracleBeam2D, construct_grids, circular_aperture
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-5
+TOL128 = 1e-10
+DEV = "cuda"
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    from lightprop2d_amd import _lib
+    _lib.load()
+
+
+def cplx(rs, shape, dt=np.complex128):
+    return (rs.standard_normal(shape) + 1j * rs.standard_normal(shape)).astype(dt)
+
+
+def to_dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def tol(dt):
+    return TOL64 if dt == np.complex64 else TOL128
+
+
+# ------------------------------------------------------------------ FFT
+FFT_SIZES = [2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096]
+
+
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128], ids=["c64", "c128"])
+@pytest.mark.parametrize("n", FFT_SIZES)
+def test_fft2_parity(n, dt):
+    from lightprop2d_amd import propagation as P
+    if n == 4096 and dt == np.complex128:
+        pytest.skip("covered by c64 at this size")
+    batch = 3 if n <= 512 else 1
+    rs = np.random.RandomState(n)
+    a = cplx(rs, (batch, n, n), dt)
+    ta = to_dev(a)
+    F = host(P.fft2(ta))
+    assert rel_l2(F, np.fft.fft2(a.astype(np.complex128))) < tol(dt) / 10
+    G = host(P.ifft2(ta))
+    assert rel_l2(G, np.fft.ifft2(a.astype(np.complex128))) < tol(dt) / 10
+    # in place
+    tb = ta.clone()
+    P.fft2(tb, out=tb)
+    assert rel_l2(host(tb), F) < 1e-12
+
+
+# ------------------------------------------------------------------ golden fixtures via Beam2D
+GOLDEN = ["gauss_propagate_n32", "speckle_aperture_n64", "evanescent_n32",
+          "lens_chain_n64", "coord_filter_n16"]
+
+
+def run_product_case(meta, z, bits):
+    from lightprop2d_amd import Beam2D
+    arrays = {k[4:]: z[k] for k in z.files if k.startswith("arr_")}
+    b = Beam2D(meta["area"], meta["n"], meta["wl"], init_field=z["field_in"], complex_bits=bits)
+    for op, arg in meta["ops"]:
+        val = arrays[arg] if isinstance(arg, str) else arg
+        if op == "coordinate_filter":
+            b.coordinate_filter(f_init=val)
+        elif op == "spectral_filter":
+            b.spectral_filter(f_init=val)
+        else:
+            getattr(b, op)(val)
+    return b
+
+
+@pytest.mark.parametrize("bits", [64, 128])
+@pytest.mark.parametrize("name", GOLDEN)
+def test_golden_beam2d(golden_dir, name, bits):
+    z = np.load(os.path.join(golden_dir, name + ".npz"))
+    meta = json.loads(str(z["meta"]))
+    b = run_product_case(meta, z, bits)
+    t = TOL64 if bits == 64 else TOL128
+    assert rel_l2(host(b.field), z["field_out"]) < t
+    assert rel_l2(host(b.spectrum), z["spectrum_out"]) < t
+
+
+# ------------------------------------------------------------------ fused propagation
+CASES = [
+    # n, area, wl, z, phase, aperture(frac of Nyquist), lens_f
+    (16, 8e-4, 632e-7, 1e-4, False, 0.0, 0.0),
+    (64, 200e-4, 632e-7, 100e-4, True, 0.0, 0.0),
+    (128, 200e-4, 532e-7, 0.1, True, 0.6, 0.0),
+    (256, 200e-4, 632e-7, 0.05, True, 0.0, 0.5),
+    (512, 500e-4, 1064e-7, 0.1, True, 0.0, 0.5),
+    (1024, 400e-4, 632e-7, 50e-4, True, 0.4, 0.0),
+    (256, 6e-4, 632e-7, 2e-4, False, 0.0, 0.0),   # evanescent modes present
+]
+
+
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128], ids=["c64", "c128"])
+@pytest.mark.parametrize("case", CASES, ids=[f"n{c[0]}-{i}" for i, c in enumerate(CASES)])
+def test_propagate_fused(case, dt):
+    from lightprop2d_amd import propagation as P
+    n, area, wl, z, use_phase, apf, lens = case
+    rs = np.random.RandomState(n + 7)
+    batch = 2 if n <= 512 else 1
+    u = cplx(rs, (batch, n, n), dt)
+    phi = rs.uniform(0, 2 * np.pi, (n, n)).astype(np.float32) if use_phase else None
+    ap = apf * 0.5 / (area / n) if apf else 0.0
+    tu = to_dev(u)
+    out = P.propagate(tu, z, area, wl, phase=to_dev(phi) if use_phase else None,
+                      aperture_radius=ap, lens_f=lens)
+    exp = np.stack([oracle.propagate_field(u[i], area, wl, z,
+                                           phase_mask=phi.astype(np.float64) if use_phase else None,
+                                           aperture_radius=ap if ap else None,
+                                           lens_f=lens if lens else None)
+                    for i in range(batch)])
+    assert rel_l2(host(out), exp) < tol(dt)
+    # in place gives the same
+    P.propagate(tu, z, area, wl, phase=to_dev(phi) if use_phase else None, aperture_radius=ap,
+                lens_f=lens, out=tu)
+    assert rel_l2(host(tu), host(out)) < 1e-12
+
+
+def test_propagate_spectrum_out():
+    from lightprop2d_amd import propagation as P
+    n, area, wl, z = 256, 300e-4, 532e-7, 0.02
+    u = cplx(np.random.RandomState(1), (n, n), np.complex64)
+    spec = torch.empty((n, n), dtype=torch.complex64, device=DEV)
+    P.propagate(to_dev(u), z, area, wl, spectrum_out=spec)
+    g = construct_grids(area, n, wl)
+    exp = np.fft.fft2(u.astype(np.complex128)) * oracle.transfer_function(g.Kz, z)
+    assert rel_l2(host(spec), exp) < TOL64
+
+
+# ------------------------------------------------------------------ adjoint
+@pytest.mark.parametrize("name", ["adjoint_n32", "adjoint_evanescent_n16"])
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128], ids=["c64", "c128"])
+def test_adjoint_golden(golden_dir, name, dt):
+    from lightprop2d_amd import propagation as P
+    z = np.load(os.path.join(golden_dir, name + ".npz"))
+    meta = json.loads(str(z["meta"]))
+    tu = to_dev(z["u"].astype(dt))
+    tphi = to_dev(z["phi"].astype(np.float32))
+    fwd = P.propagate(tu, meta["z"], meta["area"], meta["wl"], phase=tphi)
+    phi32 = z["phi"].astype(np.float32).astype(np.float64)
+    fwd_o = oracle.propagate_field(z["u"].astype(dt), meta["area"], meta["wl"], meta["z"],
+                                   phase_mask=phi32)
+    assert rel_l2(fwd_o, z["fwd"]) < 1e-6      # only the float32 rounding of the phase
+    assert rel_l2(host(fwd), fwd_o) < (TOL64 if dt == np.complex64 else 1e-9)
+    gu, gp = P.propagate_adjoint(to_dev(z["g"].astype(dt)), tu, meta["z"], meta["area"],
+                                 meta["wl"], phase=tphi)
+    # the phase was rounded to float32 on the way in: compare with the oracle at that phase
+    gu_o, gp_o = oracle.propagate_field_adjoint(z["g"].astype(dt), z["u"].astype(dt), meta["area"],
+                                                meta["wl"], meta["z"], phase_mask=phi32)
+    t = TOL64 if dt == np.complex64 else 1e-9
+    assert rel_l2(host(gu), gu_o) < t
+    # grad_phase is float32 by contract (the phase screen is float32)
+    assert rel_l2(host(gp), gp_o) < (TOL64 if dt == np.complex64 else 1e-7)
+
+
+@pytest.mark.parametrize("n", [64, 256, 1024])
+def test_adjoint_dot_product(n):
+    """<A u, g> == <u, A^H g> for the fused operator with aperture and lens."""
+    from lightprop2d_amd import propagation as P
+    rs = np.random.RandomState(n)
+    area, wl, z = 300e-4, 632e-7, 0.03
+    u = cplx(rs, (n, n))
+    g = cplx(rs, (n, n))
+    ap = 0.3 / (area / n)
+    Au = host(P.propagate(to_dev(u), z, area, wl, aperture_radius=ap, lens_f=0.7))
+    AHg, _ = P.propagate_adjoint(to_dev(g), to_dev(u), z, area, wl, aperture_radius=ap, lens_f=0.7)
+    lhs = np.vdot(g, Au)
+    rhs = np.vdot(host(AHg), u)
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+def test_adjoint_shared_phase_batch_reduction():
+    from lightprop2d_amd import propagation as P
+    n, area, wl, z = 128, 200e-4, 532e-7, 0.01
+    rs = np.random.RandomState(9)
+    u = cplx(rs, (3, n, n), np.complex64)
+    g = cplx(rs, (3, n, n), np.complex64)
+    phi = rs.uniform(0, 2 * np.pi, (n, n)).astype(np.float32)
+    gu, gp = P.propagate_adjoint(to_dev(g), to_dev(u), z, area, wl, phase=to_dev(phi))
+    exp = sum(oracle.propagate_field_adjoint(g[i], u[i], area, wl, z,
+                                             phase_mask=phi.astype(np.float64))[1] for i in range(3))
+    assert gp.shape == (n, n)
+    assert rel_l2(host(gp), exp) < TOL64
+
+
+# ------------------------------------------------------------------ grids / intensity
+@pytest.mark.parametrize("n,area,wl", [(32, 8e-4, 632e-7), (256, 200e-4, 632e-7), (1024, 6e-3, 800e-7)])
+def test_kz_grid_bit_exact(n, area, wl):
+    from lightprop2d_amd import propagation as P
+    kz = host(P.kz_grid(n, area, wl))
+    np.testing.assert_array_equal(kz, construct_grids(area, n, wl).Kz)
+
+
+def test_intensity():
+    from lightprop2d_amd import propagation as P
+    u = cplx(np.random.RandomState(2), (2, 64, 64), np.complex64)
+    np.testing.assert_allclose(host(P.intensity(to_dev(u))), np.abs(u) ** 2, rtol=1e-6)
+    u = u.astype(np.complex128)
+    np.testing.assert_allclose(host(P.intensity(to_dev(u))), np.abs(u) ** 2, rtol=1e-14)
+
+
+# ------------------------------------------------------------------ Beam2D semantics
+def test_beam2d_state_semantics():
+    from lightprop2d_amd import Beam2D, um, nm, gaussian_beam
+    b = Beam2D(200 * um, 64, 632 * nm, init_field_gen=gaussian_beam, init_gen_args=(1, 25 * um))
+    assert b.field.dtype == torch.complex128 and b.field.is_cuda
+    s0 = b.spectrum
+    b.propagate(0)
+    assert b.z == 0 and b.spectrum is s0
+    b.propagate(10 * um)
+    b.propagate(5 * um)
+    assert b.z == pytest.approx(15 * um)
+    assert b.spectrum is s0                    # multiplied in place, like the reference
+    assert isinstance(b.iprofile, np.ndarray)
+    b.numpy_output = False
+    assert isinstance(b.iprofile, torch.Tensor)
+    with pytest.raises(ValueError):
+        Beam2D(200 * um, 64, 632 * nm)
+    with pytest.raises(AttributeError):
+        Beam2D(200 * um, 64, 632 * nm, init_field=np.ones((64, 64)), complex_bits=32)
+    assert "Beam 64x64 points" in repr(b)
+
+
+def test_beam2d_filters_with_generators():
+    """coordinate_filter / spectral_filter with f_gen on the device grids vs the oracle."""
+    from lightprop2d_amd import Beam2D, round_hole, um, nm
+    n, area, wl = 256, 400 * um, 632 * nm
+    x = (np.arange(n) - n // 2) * area / n
+    f = OP.random_round_hole_phase(x, x.reshape(-1, 1), 60 * um, rng=np.random.RandomState(4))
+    a = Beam2D(area, n, wl, init_field=f)
+    o = OracleBeam2D(area, n, wl, init_field=f.copy())
+    for bm in (a, o):
+        gen = round_hole if bm is a else OP.round_hole
+        bm.coordinate_filter(f_gen=gen, fargs=(50 * um, 3 * um, -2 * um))
+        bm.propagate(80 * um)
+        bm.spectral_filter(f_gen=gen, fargs=(0.25 / wl,))
+        bm.lens(0.5)
+        bm.propagate(0.3)
+    assert rel_l2(host(a.field), o.field) < TOL128
+    assert rel_l2(host(a.spectrum), o.spectrum) < TOL128
+
+
+# ------------------------------------------------------------------ workloads
+def test_config1_gaussian_c128():
+    """256^2 c128, area 200 um, wl 632 nm, Gaussian rho0 25 um, propagate 100 um."""
+    from lightprop2d_amd import Beam2D, gaussian_beam, um, nm
+    n, area, wl, R, z = 256, 200 * um, 632 * nm, 25 * um, 100 * um
+    b = Beam2D(area, n, wl, init_field_gen=gaussian_beam, init_gen_args=(1, R))
+    o = OracleBeam2D(area, n, wl, init_field_gen=OP.gaussian_beam, init_gen_args=(1, R))
+    b.propagate(z)
+    o.propagate(z)
+    assert rel_l2(host(b.field), o.field) < TOL128
+    zR = np.pi * R ** 2 / wl
+    w = R * np.sqrt(1 + (z / zR) ** 2)
+    r2 = o.X ** 2 + o.Y ** 2
+    theory = (R / w) ** 2 * np.exp(-2 * r2 / w ** 2)
+    assert rel_l2(b.iprofile, theory) < 1e-3                      # paraxial gap ~1e-4
+
+
+def test_config2_speckle_c64():
+    """1024^2 c64, area 400 um, wl 632 nm, random-phase disc of radius 25 um (reference
+    round_hole(R) has radius R/sqrt(2), so R = 25*sqrt(2) um), seed 0; 10 x
+    [propagate(50 um), spectral aperture |f| <= NA/wl, NA 0.2]."""
+    from lightprop2d_amd import Beam2D, um, nm
+    from lightprop2d_amd import propagation as P
+    n, area, wl, NA = 1024, 400 * um, 632 * nm, 0.2
+    x = (np.arange(n) - n // 2) * area / n
+    np.random.seed(0)
+    f = OP.random_round_hole_phase(x, x.reshape(-1, 1), 25 * um * np.sqrt(2))
+    g = construct_grids(area, n, wl)
+    mask = circular_aperture(g.Kx, g.Ky, NA / wl)
+    o = OracleBeam2D(area, n, wl, init_field=f.astype(np.complex64), complex_bits=128)
+    b = Beam2D(area, n, wl, init_field=f, complex_bits=64)
+    tmask = to_dev(mask)
+    for _ in range(10):
+        o.propagate(50 * um)
+        o.spectral_filter(f_init=mask)
+        b.propagate(50 * um)
+        b.spectral_filter(f_init=tmask)
+    assert rel_l2(host(b.field), o.field) < TOL64
+    # the fused field-to-field path with the on-chip aperture gives the same field
+    u = to_dev(f.astype(np.complex64))
+    for _ in range(10):
+        P.propagate(u, 50 * um, area, wl, aperture_radius=NA / wl, out=u)
+    assert rel_l2(host(u), o.field) < TOL64
+
+
+def test_uneven_batch_vs_oracle():
+    """A scaled-down config-3 mix: 64^2, 128^2, 256^2, 512^2 fields, random z each."""
+    from lightprop2d_amd import propagation as P
+    rs = np.random.RandomState(0)
+    area, wl = 200e-4, 532e-7
+    sizes = [512] * 2 + [256] * 3 + [128] * 4 + [64] * 2
+    srcs, zs, host_in = [], [], []
+    for n in sizes:
+        x = (np.arange(n) - n // 2) * area / n
+        f = OP.random_round_hole_phase(x, x.reshape(-1, 1), 25e-4, rng=rs).astype(np.complex64)
+        host_in.append(f)
+        srcs.append(to_dev(f))
+        zs.append(rs.uniform(10e-4, 0.1))
+    dsts = [torch.empty_like(s) for s in srcs]
+    ub = P.UnevenBatch(srcs, dsts, zs, area, wl)
+    ub.run()
+    torch.cuda.synchronize()
+    for f, d, z in zip(host_in, dsts, zs):
+        assert rel_l2(host(d), oracle.propagate_field(f, area, wl, z)) < TOL64
+    # graph capture + replays give the same answer; in-place batch too
+    ref = [host(d) for d in dsts]
+    for d in dsts:
+        d.zero_()
+    gph = ub.capture_graph()
+    for _ in range(3):
+        gph.replay()
+        torch.cuda.synchronize()
+        for d, r in zip(dsts, ref):
+            assert rel_l2(host(d), r) < 1e-12
+    ip = [s.clone() for s in srcs]
+    P.UnevenBatch(ip, ip, zs, area, wl).run()
+    for d, r in zip(ip, ref):
+        assert rel_l2(host(d), r) < 1e-12
+
+
+def test_full_size_round_trip_4096():
+    """4096^2 c64 (config-5 plane size): P(z) then P(-z) is the identity on every mode
+    (H(-z) = conj H(z), evanescent H = 1), and Parseval holds without evanescent modes."""
+    from lightprop2d_amd import propagation as P
+    n, area, wl, z = 4096, 2e-1, 800e-7, 0.2
+    rs = np.random.RandomState(5)
+    u = cplx(rs, (n, n), np.complex64)
+    tu = to_dev(u)
+    v = P.propagate(tu, z, area, wl)
+    e0 = float((tu.abs() ** 2).sum())
+    e1 = float((v.abs() ** 2).sum())
+    assert e1 == pytest.approx(e0, rel=1e-5)
+    w = P.propagate(v, -z, area, wl)
+    assert rel_l2(host(w), u) < 2e-6
