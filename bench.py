@@ -118,6 +118,20 @@ def main():
         L.check(lib.lp2d_trace_enable(0))
         return el, trace_stats(lib, L)
 
+    # plain device copy of the same bytes: achievable-bandwidth reference, and the
+    # calibration dispatch for rocprof's FETCH_SIZE / WRITE_SIZE counters
+    for _ in range(3):
+        out.copy_(u)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(10):
+        out.copy_(u)
+    ev1.record()
+    torch.cuda.synchronize()
+    copy_s = ev0.elapsed_time(ev1) / 1e3 / 10
+    copy_gbs = 2 * u.numel() * 8 / copy_s / 1e9
+
     el, kt = timed(fwd_step, args.steps, args.warmup)
     props = B * args.steps
     value = props / el
@@ -137,7 +151,8 @@ def main():
                 "kernel": dom_name, "kernel_avg_us": round(avg_s * 1e6, 2),
                 "alg_bytes_per_launch": int(alg_bytes),
                 "end_to_end_GBs": round(e2e, 1),
-                "end_to_end_frac": round(e2e / HBM_PEAK_GBS, 4)}
+                "end_to_end_frac": round(e2e / HBM_PEAK_GBS, 4),
+                "copy_GBs": round(copy_gbs, 1)}
     kernels = {k: {"launches": c, "avg_us": round(ms * 1e3 / c, 2)} for k, (c, ms) in kt.items()}
 
     cpu = None

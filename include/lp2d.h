@@ -113,6 +113,15 @@ int lp2d_spectral_apply(int n, int batch, int dtype, void* spectrum, const lp2d_
 /* |field|^2 into float (C64) or double (C128) (beam2d.py:717). */
 int lp2d_intensity(int64_t count, int dtype, const void* field, void* out, void* stream);
 
+/* Intensity moments for the beam diagnostics (centroid, D4sigma; beam2d.py:669-704).
+ * Weights I = |field|^2 (computed in the field's precision, accumulated in FP64),
+ * coordinates X = (ix - n/2)*dL, Y = (iy - n/2)*dL.  For each field b, out[8*b + k]:
+ *   k=0 sum I, 1 Xc, 2 Yc, 3 sigma_x, 4 sigma_y, 5..7 scratch
+ * with Xc = sum(I X)/sum I and sigma_x = sqrt(sum(I (X - Xc)^2)/sum I) (two passes).
+ * out is a device array of 8*batch doubles; it is cleared by the call. */
+int lp2d_moments(int n, int batch, int dtype, const void* field, double dL, double* out,
+                 void* stream);
+
 /* ---------------------------------------------------------------- field -> field */
 /* field_out = ifft2(H(z) * aperture * fft2(field_in * exp(i*phase) * lens)), fused;
  * spectrum_out (may be NULL) receives H*aperture*fft2(...), i.e. the reference's
@@ -179,7 +188,9 @@ int lp2d_trace_collect(lp2d_trace_stat* stats, int max_stats, int* n_stats);
 
 /* ---------------------------------------------------------------- tuning */
 /* Number of kernel variants (schedules) available for (n, dtype, kind), kind 0 =
- * row pass, 1 = column pass; and the one in use.  Used by the autotuner. */
+ * row pass, 1 = column pass, 2 = whole-field-in-LDS kernel (0 or 1: sizes that use
+ * it run every operation in one launch); and the one in use.  Used by the
+ * autotuner. */
 int lp2d_variant_count(int n, int dtype, int kind);
 int lp2d_set_variant(int n, int dtype, int kind, int variant);
 int lp2d_get_variant(int n, int dtype, int kind);

@@ -60,6 +60,7 @@ void ensure_registry() {
   });
 }
 
+// Forward tables at [0, kTwiddleLen), conjugated ones at [kTwiddleLen, 2*kTwiddleLen).
 __global__ void k_init_twiddles(cx<float>* tf, cx<double>* td) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= kTwiddleLen - 2) return;
@@ -70,6 +71,8 @@ __global__ void k_init_twiddles(cx<float>* tf, cx<double>* td) {
   sincospi(-2.0 * (double)k / (double)N, &s, &c);
   td[e] = mk(c, s);
   tf[e] = mk((float)c, (float)s);
+  td[kTwiddleLen + e] = mk(c, -s);
+  tf[kTwiddleLen + e] = mk((float)c, (float)-s);
 }
 
 int init_device(int device) {
@@ -84,8 +87,8 @@ int init_device(int device) {
   if (prev != device && (e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
   hipStream_t s = nullptr;
   int rc = LP2D_OK;
-  if ((e = hipMalloc(&d.twf, kTwiddleLen * sizeof(cx<float>))) != hipSuccess ||
-      (e = hipMalloc(&d.twd, kTwiddleLen * sizeof(cx<double>))) != hipSuccess ||
+  if ((e = hipMalloc(&d.twf, 2 * kTwiddleLen * sizeof(cx<float>))) != hipSuccess ||
+      (e = hipMalloc(&d.twd, 2 * kTwiddleLen * sizeof(cx<double>))) != hipSuccess ||
       (e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess) {
     rc = hip_fail(e, "twiddle table allocation");
   } else {
@@ -152,6 +155,7 @@ PassArgs<T> base_args(const DeviceTables* t, int n, int batch) {
     a.tw = t->twf;
   else
     a.tw = t->twd;
+  a.tw_inv = a.tw + kTwiddleLen;
   a.fstride = (long long)n * n;
   a.batch = batch;
   a.scale = (T)1;
@@ -328,6 +332,81 @@ __global__ void k_kz(double* __restrict__ kz, int n, double k0sq, double inv_ndL
     const double kz2 = (k0sq - fx * fx) - fy * fy;
     kz[i] = kz2 >= 0.0 ? sqrt(kz2) : 0.0;
   }
+}
+
+// Intensity moments: one workgroup column (blockIdx.y) per field, grid-stride over
+// its pixels, workgroup reduction in LDS, one FP64 atomic per workgroup and moment.
+template <typename T>
+__device__ __forceinline__ double pix_intensity(const cx<T>* f, long long i) {
+  const cx<T> v = f[i];
+  const T a = v.re * v.re + v.im * v.im;  // |u|^2 in the field's precision
+  return (double)a;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
+  __syncthreads();
+  return r;
+}
+
+__global__ void k_moments_clear(double* out, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) out[i] = 0.0;
+}
+
+template <typename T>
+__global__ void k_moments(const cx<T>* __restrict__ field, int n, double dL, double* out,
+                          int pass) {
+  __shared__ double sh[16];
+  const int b = blockIdx.y;
+  const long long total = (long long)n * n;
+  const cx<T>* f = field + (long long)b * total;
+  double* o = out + 8 * b;
+  double xc = 0.0, yc = 0.0;
+  if (pass == 1) {
+    xc = o[5] / o[0];
+    yc = o[6] / o[0];
+  }
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int iy = (int)(i / n), ix = (int)(i - (long long)iy * n);
+    const double w = pix_intensity(f, i);
+    const double X = (double)(ix - n / 2) * dL, Y = (double)(iy - n / 2) * dL;
+    if (pass == 0) {
+      s0 += w;
+      s1 += w * X;
+      s2 += w * Y;
+    } else {
+      s1 += w * (X - xc) * (X - xc);
+      s2 += w * (Y - yc) * (Y - yc);
+    }
+  }
+  if (pass == 0) {
+    s0 = block_sum(s0, sh);
+    if (threadIdx.x == 0) atomicAdd(&o[0], s0);
+  }
+  s1 = block_sum(s1, sh);
+  if (threadIdx.x == 0) atomicAdd(&o[pass == 0 ? 5 : 3], s1);
+  s2 = block_sum(s2, sh);
+  if (threadIdx.x == 0) atomicAdd(&o[pass == 0 ? 6 : 4], s2);
+}
+
+__global__ void k_moments_final(double* out, int batch) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  double* o = out + 8 * b;
+  const double s = o[0];
+  o[1] = o[5] / s;
+  o[2] = o[6] / s;
+  o[3] = sqrt(o[3] / s);
+  o[4] = sqrt(o[4] / s);
 }
 
 unsigned grid_for(long long total) {
@@ -559,6 +638,28 @@ int lp2d_kz_grid(int n, const lp2d_optics* o, double* kz, void* stream) {
   return e == hipSuccess ? LP2D_OK : hip_fail(e, "kz kernel launch");
 }
 
+int lp2d_moments(int n, int batch, int dtype, const void* field, double dL, double* out,
+                 void* stream) {
+  int lg;
+  if (int rc = check_n(n, batch, dtype, &lg)) return rc;
+  if (!field || !out) return fail(LP2D_E_ARG, "null pointer");
+  if (!(dL > 0)) return fail(LP2D_E_ARG, "dL must be > 0");
+  hipStream_t s = (hipStream_t)stream;
+  k_moments_clear<<<(8 * batch + 255) / 256, 256, 0, s>>>(out, 8 * batch);
+  long long per = ((long long)n * n + 255) / 256;
+  const unsigned gx = (unsigned)(per < 1024 ? per : 1024);
+  const dim3 grid(gx, (unsigned)batch);
+  for (int pass = 0; pass < 2; ++pass) {
+    if (dtype == LP2D_C64)
+      k_moments<float><<<grid, 256, 0, s>>>((const cx<float>*)field, n, dL, out, pass);
+    else
+      k_moments<double><<<grid, 256, 0, s>>>((const cx<double>*)field, n, dL, out, pass);
+  }
+  k_moments_final<<<(batch + 63) / 64, 64, 0, s>>>(out, batch);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? LP2D_OK : hip_fail(e, "moments kernel launch");
+}
+
 int lp2d_fft2(int n, int batch, int dtype, const void* in, void* out, int inverse, void* stream) {
   int lg;
   if (int rc = check_n(n, batch, dtype, &lg)) return rc;
@@ -663,12 +764,14 @@ int lp2d_sum_planes(int64_t count, int batch, const float* in, float* out, void*
 int lp2d_variant_count(int n, int dtype, int kind) {
   ensure_registry();
   const int lg = log2_exact(n);
-  if (lg < 1 || lg > kLogMax || (dtype != 0 && dtype != 1) || (kind != 0 && kind != 1)) return 0;
+  if (lg < 1 || lg > kLogMax || (dtype != 0 && dtype != 1) || kind < 0 || kind > 2) return 0;
+  if (kind == 2) return dtype == 0 ? (int)g_rf->has_small[lg] : (int)g_rd->has_small[lg];
   if (dtype == 0) return kind == 0 ? g_rf->nrows[lg] : g_rf->ncols[lg];
   return kind == 0 ? g_rd->nrows[lg] : g_rd->ncols[lg];
 }
 
 int lp2d_set_variant(int n, int dtype, int kind, int variant) {
+  if (kind != 0 && kind != 1) return fail(LP2D_E_ARG, "kind must be 0 (rows) or 1 (columns)");
   const int cnt = lp2d_variant_count(n, dtype, kind);
   if (cnt == 0) return fail(LP2D_E_SIZE, "no variants for n=%d dtype=%d kind=%d", n, dtype, kind);
   if (variant < 0 || variant >= cnt) return fail(LP2D_E_ARG, "variant %d out of range [0,%d)", variant, cnt);
@@ -677,13 +780,15 @@ int lp2d_set_variant(int n, int dtype, int kind, int variant) {
 }
 
 int lp2d_get_variant(int n, int dtype, int kind) {
-  if (lp2d_variant_count(n, dtype, kind) == 0) return fail(LP2D_E_SIZE, "no variants");
+  if ((kind != 0 && kind != 1) || lp2d_variant_count(n, dtype, kind) == 0)
+    return fail(LP2D_E_SIZE, "no variants");
   return g_sel[dtype][kind][log2_exact(n)];
 }
 
 int lp2d_variant_name(int n, int dtype, int kind, int variant, char* buf, int buflen) {
   const int cnt = lp2d_variant_count(n, dtype, kind);
-  if (variant < 0 || variant >= cnt || !buf || buflen < 1) return fail(LP2D_E_ARG, "bad variant query");
+  if (kind == 2 || variant < 0 || variant >= cnt || !buf || buflen < 1)
+    return fail(LP2D_E_ARG, "bad variant query");
   const int lg = log2_exact(n);
   if (kind == 0) {
     const int E = dtype ? g_rd->rows[lg][variant].E : g_rf->rows[lg][variant].E;

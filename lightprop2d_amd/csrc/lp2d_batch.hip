@@ -74,6 +74,7 @@ __device__ __forceinline__ PassArgs<float> field_args(const FieldDev& f, const c
   PassArgs<float> a;
   memset(&a, 0, sizeof(a));
   a.tw = tw;
+  a.tw_inv = tw + kTwiddleLen;
   a.src = f.src;
   a.dst = f.dst;
   a.fstride = 0;

@@ -194,7 +194,8 @@ struct StrideAcc {
 // registers v[q + r*E/R].  The last pass writes back into registers in the stride-TL
 // layout, so only the passes before it exchange through LDS.
 //
-// tw points at the twiddle table of length N: tw[i] = exp(-2*pi*i*i/N).
+// tw points at the twiddle table of length N for this direction:
+// tw[i] = exp(-+2*pi*i*i/N) (minus for the forward transform).
 template <typename T, int N, int E, bool INV, int NS, typename Acc>
 __device__ __forceinline__ void stockham(cx<T> (&v)[E], int t, const Acc& acc,
                                          const cx<T>* __restrict__ tw) {
@@ -215,8 +216,7 @@ __device__ __forceinline__ void stockham(cx<T> (&v)[E], int t, const Acc& acc,
       constexpr int TS = N / (NS * R);
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        const cx<T> w = tw[k * r * TS];
-        a[r] = INV ? cmulc(a[r], w) : cmul(a[r], w);
+        a[r] = cmul(a[r], tw[k * r * TS]);
       }
     }
     dft<T, R, INV>(a);

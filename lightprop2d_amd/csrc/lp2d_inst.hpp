@@ -136,17 +136,25 @@ void register_n(Registry<T>& r) {
   constexpr int EMAX = F32 ? 16 : 8;
   constexpr int E0 = cmin(N, EMAX);
   constexpr int E1 = cmin(N, EMAX / 2);
+  constexpr int WT = F32 ? 16 : 8;  // 128-byte row segments
   r.nrows[lg] = 0;
   r.ncols[lg] = 0;
   add_row<T, N, E0>(r);
-  add_col<T, N, E0, F32 ? 16 : 8>(r);
+  add_col<T, N, E0, WT>(r);
   if constexpr (E1 != E0 && N >= 16) {
     add_row<T, N, E1>(r);
-    add_col<T, N, E1, F32 ? 16 : 8>(r);
+    add_col<T, N, E1, WT>(r);
   }
-  if constexpr (F32 && N >= 256) {
-    add_row<T, N, 32>(r);
-    add_col<T, N, 32, 16>(r);
+  if constexpr (N >= 256) {
+    add_col<T, N, E0, WT / 2>(r);
+    if constexpr (F32) {
+      add_row<T, N, 32>(r);
+      add_col<T, N, 32, 16>(r);
+      add_col<T, N, 32, 8>(r);
+    } else {
+      add_row<T, N, 16>(r);
+      add_col<T, N, 16, 8>(r);
+    }
   }
   using SS = SmallShape<T, N, EMAX>;
   if constexpr (SS::OK) {

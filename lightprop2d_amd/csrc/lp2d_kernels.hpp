@@ -49,7 +49,8 @@ struct MaskD {
 
 template <typename T>
 struct PassArgs {
-  const cx<T>* tw;  // concatenated twiddle tables
+  const cx<T>* tw;      // concatenated twiddle tables, exp(-2*pi*i*k/N)
+  const cx<T>* tw_inv;  // the same, conjugated (inverse passes)
   const cx<T>* src;
   cx<T>* dst;
   long long fstride;  // elements between consecutive fields
@@ -81,7 +82,10 @@ struct PassArgs {
 
 // ----------------------------------------------------------------- twiddles
 // Concatenated per-size tables (PassArgs::tw): the table of size N (a power of two)
-// starts at N - 2 and holds exp(-2*pi*i*k/N), k < N.
+// starts at N - 2 and holds exp(-2*pi*i*k/N), k < N; PassArgs::tw_inv holds the
+// conjugates.  Separate tables keep the forward and inverse twiddle loads distinct,
+// so a kernel running both transforms never holds the forward ones live across the
+// spectral stage.
 constexpr int kMaxN = 8192;
 constexpr int kTwiddleLen = 2 * kMaxN;
 
@@ -263,7 +267,7 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
     }
   }
   LineAcc<T, PS> acc{lds + l * L};
-  line_fft<T, N, E, INV>(v, t, acc, (a.tw + (N - 2)));
+  line_fft<T, N, E, INV>(v, t, acc, (INV ? a.tw_inv : a.tw) + (N - 2));
   cx<T>* __restrict__ dst = a.dst + frow;
 #pragma unroll
   for (int m = 0; m < E; ++m) {
@@ -293,8 +297,7 @@ __device__ __forceinline__ void cols_body(const PassArgs<T>& a, int b, int x0, c
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = src[(long long)(t + TL * m) * N];
   InterAcc<T, PS, W> acc{lds + c};
-  const cx<T>* tw = (a.tw + (N - 2));
-  if constexpr ((MODE & MODE_FWD) != 0) line_fft<T, N, E, false>(v, t, acc, tw);
+  if constexpr ((MODE & MODE_FWD) != 0) line_fft<T, N, E, false>(v, t, acc, a.tw + (N - 2));
   if (a.flags & F_MID) {
 #pragma unroll
     for (int m = 0; m < E; ++m) {
@@ -303,7 +306,7 @@ __device__ __forceinline__ void cols_body(const PassArgs<T>& a, int b, int x0, c
       if (a.flags & F_SPEC_OUT) a.spec_out[fbase + (long long)ky * N] = v[m];
     }
   }
-  if constexpr ((MODE & MODE_INV) != 0) line_fft<T, N, E, true>(v, t, acc, tw);
+  if constexpr ((MODE & MODE_INV) != 0) line_fft<T, N, E, true>(v, t, acc, a.tw_inv + (N - 2));
   cx<T>* __restrict__ dst = a.dst + fbase;
   const bool sc = (a.flags & F_SCALE) != 0;
 #pragma unroll
@@ -328,7 +331,8 @@ __device__ __forceinline__ void small_body(const PassArgs<T>& a, int b, int mode
                                            int tid) {
   constexpr int TL = N / E;
   constexpr int P = SmallPitch<N, PS>::value;
-  const cx<T>* tw = (a.tw + (N - 2));
+  const cx<T>* tw = a.tw + (N - 2);
+  const cx<T>* twi = a.tw_inv + (N - 2);
   const long long fb = (long long)b * a.fstride;
   cx<T> v[E];
   // phase 1: rows (load, spatial prologue, forward row FFT)
@@ -367,7 +371,7 @@ __device__ __forceinline__ void small_body(const PassArgs<T>& a, int b, int mode
         if (a.flags & F_SPEC_OUT) a.spec_out[fb + (long long)ky * N + x] = v[m];
       }
     }
-    if (mode & MODE_INV) line_fft<T, N, E, true>(v, t, acc, tw);
+    if (mode & MODE_INV) line_fft<T, N, E, true>(v, t, acc, twi);
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < E; ++m) acc.st(t + TL * m, v[m]);
@@ -381,7 +385,7 @@ __device__ __forceinline__ void small_body(const PassArgs<T>& a, int b, int mode
     for (int m = 0; m < E; ++m) v[m] = acc.ld(t + TL * m);
     if (mode & MODE_INV) {
       __syncthreads();
-      line_fft<T, N, E, true>(v, t, acc, tw);
+      line_fft<T, N, E, true>(v, t, acc, twi);
     }
     cx<T>* __restrict__ dst = a.dst + fb + (long long)y * N;
 #pragma unroll

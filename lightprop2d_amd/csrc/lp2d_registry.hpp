@@ -31,7 +31,7 @@ struct SmallVar {
   int E, threads, lds;
 };
 
-constexpr int kMaxVariants = 4;
+constexpr int kMaxVariants = 6;
 constexpr int kLogMax = 13;  // n <= 8192
 
 template <typename T>

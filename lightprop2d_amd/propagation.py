@@ -25,6 +25,28 @@ _MASK_DT = {torch.bool: L.LP2D_MASK_U8, torch.uint8: L.LP2D_MASK_U8,
             torch.complex64: L.LP2D_MASK_C64, torch.complex128: L.LP2D_MASK_C128}
 
 
+_tuned = False
+
+
+def _lib():
+    """Load liblp2d and, on first use, install the tuned kernel variants."""
+    global _tuned
+    lib = L.lib()
+    if not _tuned:
+        _tuned = True
+        from . import autotune
+        try:
+            autotune.apply_defaults(autotune.arch())
+        except L.Lp2dError:
+            pass
+    return lib
+
+
+def _small_path(n: int, dtype: str = "c64") -> bool:
+    """True when size n runs every operation in one whole-field-in-LDS launch."""
+    return L.lib().lp2d_variant_count(n, L.LP2D_C64 if dtype == "c64" else L.LP2D_C128, 2) == 1
+
+
 def _stream(t: torch.Tensor):
     return C.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
 
@@ -92,7 +114,7 @@ def _masks(masks: Sequence, n: int, batch: int, device):
 
 def kz_grid(n: int, area_size: float, wl: float, device=None) -> torch.Tensor:
     """Kz = nan_to_num(sqrt(k0^2 - Kx^2 - Ky^2)), FP64 (n, n) (beam2d.py:304-308)."""
-    lib = L.lib()
+    lib = _lib()
     dev = torch.device(device) if device is not None else torch.device("cuda")
     out = torch.empty((n, n), dtype=torch.float64, device=dev)
     o = optics(area_size, wl)
@@ -103,7 +125,7 @@ def kz_grid(n: int, area_size: float, wl: float, device=None) -> torch.Tensor:
 
 def fft2(field: torch.Tensor, out: Optional[torch.Tensor] = None, inverse: bool = False):
     """2-D FFT over the last two axes, numpy normalisation (beam2d.py:207-229)."""
-    lib = L.lib()
+    lib = _lib()
     n, batch, dt = _field_geom(field)
     if out is None:
         out = torch.empty_like(field)
@@ -123,7 +145,7 @@ def spatial_apply(field: torch.Tensor, masks: Sequence = (), lens_f: float = 0.0
                   area_size: float = 0.0, wl: float = 0.0, phase: Optional[torch.Tensor] = None,
                   spectrum_out: Optional[torch.Tensor] = None):
     """field *= masks * lens * exp(i*phase) in place; spectrum_out = fft2(field)."""
-    lib = L.lib()
+    lib = _lib()
     n, batch, dt = _field_geom(field)
     if len(masks) > 2:
         raise Lp2dError("at most two masks per call")
@@ -144,7 +166,7 @@ def spectral_apply(spectrum: torch.Tensor, masks: Sequence = (), z: float = 0.0,
                    area_size: float = 0.0, wl: float = 0.0, aperture_radius: float = 0.0,
                    evanescent: str = "reference", field_out: Optional[torch.Tensor] = None):
     """spectrum *= H(z) * aperture * masks in place; field_out = ifft2(spectrum)."""
-    lib = L.lib()
+    lib = _lib()
     n, batch, dt = _field_geom(spectrum, "spectrum")
     if len(masks) > 2:
         raise Lp2dError("at most two masks per call")
@@ -187,7 +209,7 @@ def propagate(field: torch.Tensor, z: float, area_size: float, wl: float,
     modes pass unchanged unless `evanescent` is "zero" or "decay".  `out` may be
     `field` (in place).  Not differentiable; see lightprop2d_amd.autograd.
     """
-    lib = L.lib()
+    lib = _lib()
     n, batch, dt = _field_geom(field)
     if out is None:
         out = torch.empty_like(field)
@@ -211,7 +233,7 @@ def propagate_adjoint(grad_out: torch.Tensor, field_in: Optional[torch.Tensor], 
 
     Returns (grad_field, grad_phase) where grad_phase has the phase's shape (summed
     over the batch for a shared (n, n) screen) or is None."""
-    lib = L.lib()
+    lib = _lib()
     n, batch, dt = _field_geom(grad_out, "grad_out")
     if out is None:
         out = torch.empty_like(grad_out)
@@ -240,7 +262,7 @@ def propagate_adjoint(grad_out: torch.Tensor, field_in: Optional[torch.Tensor], 
 
 def sum_planes(planes: torch.Tensor, out: torch.Tensor):
     """out = planes.sum(0) for a float32 (batch, ...) tensor, in a HIP kernel."""
-    lib = L.lib()
+    lib = _lib()
     b = planes.shape[0]
     cnt = planes[0].numel()
     L.check(lib.lp2d_sum_planes(cnt, b, _ptr(planes), _ptr(out), _stream(planes)), "sum_planes")
@@ -249,7 +271,7 @@ def sum_planes(planes: torch.Tensor, out: torch.Tensor):
 
 def intensity(field: torch.Tensor, out: Optional[torch.Tensor] = None):
     """|field|^2 (beam2d.py:717): float32 for complex64, float64 for complex128."""
-    lib = L.lib()
+    lib = _lib()
     if field.dtype not in _DT or field.device.type != "cuda" or not field.is_contiguous():
         raise Lp2dError("intensity needs a contiguous complex tensor on a HIP device")
     rdt = torch.float32 if field.dtype == torch.complex64 else torch.float64
@@ -272,7 +294,7 @@ class UnevenBatch:
 
     def __init__(self, srcs: Sequence[torch.Tensor], dsts: Sequence[torch.Tensor],
                  zs: Iterable[float], area_size, wl, aperture_radius=0.0):
-        lib = L.lib()
+        lib = _lib()
         srcs = list(srcs)
         dsts = list(dsts)
         zs = list(zs)

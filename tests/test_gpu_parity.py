@@ -358,3 +358,53 @@ def test_full_size_round_trip_4096():
     assert e1 == pytest.approx(e0, rel=1e-5)
     w = P.propagate(v, -z, area, wl)
     assert rel_l2(host(w), u) < 2e-6
+
+
+# ------------------------------------------------------------------ diagnostics (beam2d.py:669-752)
+@pytest.mark.parametrize("bits", [64, 128])
+def test_diagnostics_vs_oracle(bits):
+    from lightprop2d_amd import Beam2D, gaussian_beam, um, nm
+    n, area, wl = 256, 400 * um, 632 * nm
+    b = Beam2D(area, n, wl, init_field_gen=gaussian_beam, init_gen_args=(1, 30 * um, 20 * um, -15 * um),
+               complex_bits=bits)
+    o = OracleBeam2D(area, n, wl, init_field_gen=OP.gaussian_beam,
+                     init_gen_args=(1, 30 * um, 20 * um, -15 * um), complex_bits=bits)
+    b.propagate(0.05)
+    o.propagate(0.05)
+    cb, co = b.centroid, o.centroid
+    assert cb[2:] == co[2:]
+    assert cb[0] == pytest.approx(co[0], rel=1e-9 if bits == 128 else 1e-5)
+    assert cb[1] == pytest.approx(co[1], rel=1e-9 if bits == 128 else 1e-5)
+    assert b.D4sigma == pytest.approx(o.D4sigma, rel=1e-9 if bits == 128 else 1e-5)
+    assert b.centroid_intensity == pytest.approx(o.centroid_intensity, rel=1e-9 if bits == 128 else 1e-5)
+    # phase is compared where the beam has power (elsewhere it is the phase of noise)
+    live = o.iprofile > 1e-6 * o.iprofile.max()
+    d = np.angle(np.exp(1j * (b.phiprofile[live] - o.phiprofile[live])))
+    assert np.abs(d).max() < (1e-9 if bits == 128 else 1e-3)
+
+
+def test_reference_gaussian_diffraction_script_on_gpu():
+    """The reference's analytic check (tests/gaussbeam_diffraction.py:18-79) run through
+    the GPU Beam2D: 512^2, area 2 mm, 532 nm, R = 0.1 mm, 2 mm steps; on-axis
+    intensity 1/(1+(z/zR)^2), width w/w0 from D4sigma/2, Gouy phase arctan(z/zR)."""
+    from lightprop2d_amd import Beam2D, gaussian_beam, mm, um
+    npoints, area_size, wl0, R = 512, 2 * mm, 0.532 * um, 0.1 * mm
+    beam = Beam2D(area_size, npoints, wl0, init_field_gen=gaussian_beam, init_gen_args=(1, R),
+                  use_gpu=True)
+    dz = 2 * mm
+    z_grid = np.arange(0, 40, dz)[:8]
+    intensities, widths, phases = [], [], []
+    beam.propagate(z_grid[0])
+    for z in z_grid:
+        if z > z_grid[0]:
+            beam.propagate(dz)
+        intensities.append(beam.centroid_intensity)
+        widths.append(beam.D4sigma[0] / 2)
+        phases.append(beam.phiprofile[beam.centroid[3], beam.centroid[2]])
+    zn = 2 * z_grid / (2 * np.pi * beam.k0 * R ** 2)
+    np.testing.assert_allclose(np.array(intensities) / intensities[0], 1 / (1 + zn ** 2), rtol=2e-3)
+    np.testing.assert_allclose(np.array(widths) / R, np.sqrt(1 + (wl0 * z_grid / np.pi / R ** 2) ** 2),
+                               rtol=2e-3)
+    ph = np.array(phases) - 2 * np.pi * (np.trunc(beam.k0 * z_grid) - beam.k0 * z_grid)
+    ph = (ph + np.pi) % (2 * np.pi) - np.pi
+    np.testing.assert_allclose(ph, np.arctan(wl0 * z_grid / np.pi / R ** 2), atol=2e-3)
