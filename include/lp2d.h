@@ -136,13 +136,19 @@ int lp2d_propagate(int n, int batch, int dtype, const void* field_in, void* fiel
  * PyTorch's complex-gradient convention (grad = dL/dRe + i dL/dIm):
  *   g1 = ifft2(conj(H) * aperture * fft2(grad_out))
  *   grad_in = conj(exp(i*phase) * lens) * g1
- *   grad_phase[b] = Im(conj(field_in[b] * exp(i*phase) * lens) * g1[b])   (per field)
+ *   gphi[b] = Im(conj(field_in[b] * exp(i*phase) * lens) * g1[b])
  * grad_phase (float32, (batch, n, n)) may be NULL, and must be NULL when phase is.
+ * With a per-field phase (phase_stride_b = n*n) plane b receives gphi[b]; with a
+ * shared screen (phase_stride_b = 0) plane 0 receives sum_b gphi[b] and the other
+ * planes are scratch.  grad_in is always required (the passes use it as their
+ * intermediate plane); with LP2D_ADJ_NO_GRAD_IN in adj_flags its contents are
+ * undefined on return (for callers that only need the phase gradient).
  * No counterpart in the reference (it has no gradients). */
+#define LP2D_ADJ_NO_GRAD_IN 1
 int lp2d_propagate_adjoint(int n, int batch, int dtype, const void* grad_out,
                            const void* field_in, const float* phase, int64_t phase_stride_b,
                            const lp2d_optics* optics, void* grad_in, float* grad_phase,
-                           void* stream);
+                           int adj_flags, void* stream);
 
 /* Sum `batch` float planes of `count` elements: out[i] = sum_b in[b*count + i].
  * Used to reduce per-field phase gradients of a shared mask. */

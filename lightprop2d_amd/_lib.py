@@ -30,6 +30,8 @@ LP2D_MASK_C128 = 5
 
 EVANESCENT = {"reference": 0, "pass": 0, "zero": 1, "decay": 2}
 
+LP2D_ADJ_NO_GRAD_IN = 1
+
 # every symbol include/lp2d.h declares
 EXPORTED = [
     "lp2d_init", "lp2d_last_error", "lp2d_version", "lp2d_max_n", "lp2d_kz_grid",
@@ -92,7 +94,7 @@ _SIGS = {
     "lp2d_intensity": (_I, [_I64, _I, _P, _P, _P]),
     "lp2d_propagate": (_I, [_I, _I, _I, _P, _P, _P, _I64, C.POINTER(lp2d_optics), _P, _P]),
     "lp2d_propagate_adjoint": (_I, [_I, _I, _I, _P, _P, _P, _I64, C.POINTER(lp2d_optics),
-                                    _P, _P, _P]),
+                                    _P, _P, _I, _P]),
     "lp2d_sum_planes": (_I, [_I64, _I, _P, _P, _P]),
     "lp2d_batch_create": (_I, [C.POINTER(lp2d_field_desc), _I, _I, C.POINTER(_P)]),
     "lp2d_batch_run": (_I, [_P, _P]),

@@ -50,7 +50,8 @@ class PropagateFn(torch.autograd.Function):
         gu, gphi = P.propagate_adjoint(grad_out.contiguous(), u, z, area, wl,
                                        phase=phase if ctx.has_phase else None,
                                        aperture_radius=ap, lens_f=lens, evanescent=ev,
-                                       want_phase_grad=bool(need_phase and ctx.has_phase))
+                                       want_phase_grad=bool(need_phase and ctx.has_phase),
+                                       want_field_grad=bool(need_u))
         return (gu if need_u else None, gphi if need_phase else None,
                 None, None, None, None, None, None)
 
@@ -129,8 +130,10 @@ class StackFn(torch.autograd.Function):
                     planes.append(advance(planes[-1], k, k + 1))
             for k in reversed(range(seg0, seg1)):
                 uk = planes[k - seg0] if planes is not None else advance(ck, seg0, k)
+                need_g = k > 0 or ctx.needs_input_grad[0]
                 g, gphi = P.propagate_adjoint(g, uk, dz, area, wl, phase=screens[k],
-                                              want_phase_grad=gscreens is not None)
+                                              want_phase_grad=gscreens is not None,
+                                              want_field_grad=need_g)
                 if gscreens is not None:
                     gscreens[k].copy_(gphi)
                 del uk

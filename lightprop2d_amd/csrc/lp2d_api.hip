@@ -54,8 +54,10 @@ void ensure_registry() {
     std::memset(g_rd, 0, sizeof(Registry<double>));
     register_f32_a(*g_rf);
     register_f32_b(*g_rf);
+    register_f32_c(*g_rf);
     register_f64_a(*g_rd);
     register_f64_b(*g_rd);
+    register_f64_c(*g_rd);
     std::memset(g_sel, 0, sizeof(g_sel));
   });
 }
@@ -310,8 +312,8 @@ __global__ void k_intensity(const cx<T>* __restrict__ f, T* __restrict__ out, lo
   }
 }
 
-__global__ void k_sum_planes(const float* __restrict__ in, float* __restrict__ out, long long count,
-                             int batch) {
+// out may alias plane 0 of in: each thread reads all planes of its element first.
+__global__ void k_sum_planes(const float* in, float* out, long long count, int batch) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count;
        i += (long long)gridDim.x * blockDim.x) {
     float s = 0.f;
@@ -576,14 +578,49 @@ int op_propagate(int n, int lg, int batch, const void* in, void* out, const floa
 }
 
 template <typename T>
+int run_rows_grad(const PassArgs<T>& a, int lg, float* gphase, hipStream_t s) {
+  // the gradient pass carries the screen factor and the accumulators: use the
+  // schedule with the fewest elements per thread (lowest register pressure)
+  int vi = 0;
+  for (int i = 1; i < reg<T>()->nrows[lg]; ++i)
+    if (reg<T>()->rows[lg][i].E < reg<T>()->rows[lg][vi].E) vi = i;
+  const RowVar<T>& v = reg<T>()->rows[lg][vi];
+  const int n = 1 << lg;
+  const int tpf = (n + v.lines - 1) / v.lines;
+  int groups = 2048 / tpf;
+  if (groups < 1) groups = 1;
+  if (groups > a.batch) groups = a.batch;
+  std::string nm;
+  if (g_trace_on) {
+    char b[64];
+    snprintf(b, sizeof(b), "rows_grad n=%d %s E=%d", n, tname<T>(), v.E);
+    nm = b;
+  }
+  hipError_t e;
+  {
+    TraceScope ts(s, nm);
+    e = v.grad_fn(a, groups, gphase, s);
+  }
+  if (e != hipSuccess) return hip_fail(e, "adjoint gradient pass launch");
+  if (groups > 1) {
+    const long long cnt = (long long)n * n;
+    k_sum_planes<<<grid_for(cnt), 256, 0, s>>>(gphase, gphase, cnt, groups);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "sum kernel launch");
+  }
+  return LP2D_OK;
+}
+
+template <typename T>
 int op_adjoint(int n, int lg, int batch, const void* gout, const void* u, const float* phase,
-               int64_t phase_sb, const lp2d_optics* o, void* gin, float* gphase, hipStream_t s) {
+               int64_t phase_sb, const lp2d_optics* o, void* gin, float* gphase, int aflags,
+               hipStream_t s) {
   DeviceTables* t;
   if (int rc = current_tables(&t)) return rc;
   PassArgs<T> a = base_args<T>(t, n, batch);
   set_optics(a, n, o);
   a.z = -o->z;  // conj(H(z)) = H(-z) under the reference's trunc rule; evanescent stays 1
-  int mid = 0, epi = F_SCALE;
+  const int no_store = (aflags & LP2D_ADJ_NO_GRAD_IN) ? F_NO_STORE : 0;
+  int mid = 0, epi = F_SCALE | no_store;
   if (o->z != 0.0) mid |= F_H;
   if (o->aperture_radius > 0.0) mid |= F_APERTURE;
   if (phase || o->lens_f != 0.0) {
@@ -594,21 +631,34 @@ int op_adjoint(int n, int lg, int batch, const void* gout, const void* u, const 
     a.grad_phase = gphase;
     epi |= F_GRAD;
   }
+  const bool shared_sum = gphase && phase_sb == 0 && batch > 1;
   const T sc = (T)(1.0 / ((double)n * n));
   a.scale = sc;
   a.src = (const cx<T>*)gout;
   a.dst = (cx<T>*)gin;
   if (use_small<T>(lg)) {
     a.flags = mid | epi;
-    return run_small<T>(a, lg, MODE_FWD | MODE_INV, s);
+    if (int rc = run_small<T>(a, lg, MODE_FWD | MODE_INV, s)) return rc;
+  } else {
+    a.flags = 0;
+    if (int rc = run_rows<T>(a, lg, false, s)) return rc;
+    a.src = (const cx<T>*)gin;
+    a.flags = mid;
+    if (int rc = run_cols<T>(a, lg, MODE_FWD | MODE_INV, s)) return rc;
+    if (shared_sum) {
+      a.flags = F_SCALE | no_store;
+      return run_rows_grad<T>(a, lg, gphase, s);
+    }
+    a.flags = epi;
+    if (int rc = run_rows<T>(a, lg, true, s)) return rc;
   }
-  a.flags = 0;
-  if (int rc = run_rows<T>(a, lg, false, s)) return rc;
-  a.src = (const cx<T>*)gin;
-  a.flags = mid;
-  if (int rc = run_cols<T>(a, lg, MODE_FWD | MODE_INV, s)) return rc;
-  a.flags = epi;
-  return run_rows<T>(a, lg, true, s);
+  if (shared_sum) {  // per-field planes from the whole-field kernel: reduce into plane 0
+    const long long cnt = (long long)n * n;
+    k_sum_planes<<<grid_for(cnt), 256, 0, s>>>(gphase, gphase, cnt, batch);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "sum kernel launch");
+  }
+  return LP2D_OK;
 }
 
 }  // namespace
@@ -737,7 +787,7 @@ int lp2d_propagate(int n, int batch, int dtype, const void* field_in, void* fiel
 
 int lp2d_propagate_adjoint(int n, int batch, int dtype, const void* grad_out, const void* field_in,
                            const float* phase, int64_t phase_stride_b, const lp2d_optics* optics,
-                           void* grad_in, float* grad_phase, void* stream) {
+                           void* grad_in, float* grad_phase, int adj_flags, void* stream) {
   int lg;
   if (int rc = check_n(n, batch, dtype, &lg)) return rc;
   if (!grad_out || !grad_in || !optics) return fail(LP2D_E_ARG, "null pointer");
@@ -746,10 +796,11 @@ int lp2d_propagate_adjoint(int n, int batch, int dtype, const void* grad_out, co
   if ((phase || optics->lens_f != 0.0) && !field_in)
     return fail(LP2D_E_ARG, "field_in is required with a phase screen or lens");
   if (field_in && grad_in == field_in) return fail(LP2D_E_ARG, "grad_in must not alias field_in");
+  if (adj_flags & ~LP2D_ADJ_NO_GRAD_IN) return fail(LP2D_E_ARG, "unknown adj_flags bits");
   hipStream_t s = (hipStream_t)stream;
   return dtype == LP2D_C64
-             ? op_adjoint<float>(n, lg, batch, grad_out, field_in, phase, phase_stride_b, optics, grad_in, grad_phase, s)
-             : op_adjoint<double>(n, lg, batch, grad_out, field_in, phase, phase_stride_b, optics, grad_in, grad_phase, s);
+             ? op_adjoint<float>(n, lg, batch, grad_out, field_in, phase, phase_stride_b, optics, grad_in, grad_phase, adj_flags, s)
+             : op_adjoint<double>(n, lg, batch, grad_out, field_in, phase, phase_stride_b, optics, grad_in, grad_phase, adj_flags, s);
 }
 
 int lp2d_sum_planes(int64_t count, int batch, const float* in, float* out, void* stream) {

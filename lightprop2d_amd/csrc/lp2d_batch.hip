@@ -94,12 +94,12 @@ __device__ __forceinline__ void row_tile(PassArgs<float>& a, int tile, bool inv,
     a.src = a.dst;
     a.flags = F_SCALE;
     a.scale = 1.0f / ((float)N * (float)N);
-    rows_body<float, N, kEb, S::LINES, true, S::PS>(a, 0, tile * S::LINES, S::LINES, lds,
-                                                   threadIdx.x);
+    rows_body<float, N, kEb, S::LINES, true, S::PS, false>(a, 0, tile * S::LINES, S::LINES,
+                                                          lds, threadIdx.x);
   } else {
     a.flags = 0;
-    rows_body<float, N, kEb, S::LINES, false, S::PS>(a, 0, tile * S::LINES, S::LINES, lds,
-                                                    threadIdx.x);
+    rows_body<float, N, kEb, S::LINES, false, S::PS, false>(a, 0, tile * S::LINES, S::LINES,
+                                                           lds, threadIdx.x);
   }
 }
 
@@ -109,8 +109,8 @@ __device__ __forceinline__ void col_tile(PassArgs<float>& a, int tile, int mid,
   using S = TileShape<N>;
   a.src = a.dst;
   a.flags = mid;
-  cols_body<float, N, kEb, S::LINES, MODE_FWD | MODE_INV, S::PS>(a, 0, tile * S::LINES, lds,
-                                                                threadIdx.x);
+  cols_body<float, N, kEb, S::LINES, MODE_FWD | MODE_INV, S::PS, 1>(a, 0, tile * S::LINES,
+                                                                   lds, threadIdx.x);
 }
 
 __global__ __launch_bounds__(kThreads) void k_batch_sweep(BatchArgs ba) {

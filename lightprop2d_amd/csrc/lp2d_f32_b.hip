@@ -1,4 +1,4 @@
-// Kernel instantiations: float, n = 2^9 .. 2^13.
+// Kernel instantiations: float, n = 2^9 .. 2^10.
 #include "lp2d_inst.hpp"
 
 namespace lp2d {
@@ -6,9 +6,6 @@ namespace lp2d {
 void register_f32_b(Registry<float>& r) {
   register_n<float, 512>(r);
   register_n<float, 1024>(r);
-  register_n<float, 2048>(r);
-  register_n<float, 4096>(r);
-  register_n<float, 8192>(r);
 }
 
 }  // namespace lp2d

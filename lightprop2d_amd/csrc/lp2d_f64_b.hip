@@ -1,4 +1,4 @@
-// Kernel instantiations: double, n = 2^9 .. 2^13.
+// Kernel instantiations: double, n = 2^9 .. 2^10.
 #include "lp2d_inst.hpp"
 
 namespace lp2d {
@@ -6,9 +6,6 @@ namespace lp2d {
 void register_f64_b(Registry<double>& r) {
   register_n<double, 512>(r);
   register_n<double, 1024>(r);
-  register_n<double, 2048>(r);
-  register_n<double, 4096>(r);
-  register_n<double, 8192>(r);
 }
 
 }  // namespace lp2d

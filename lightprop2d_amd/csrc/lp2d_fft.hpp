@@ -214,9 +214,15 @@ __device__ __forceinline__ void stockham(cx<T> (&v)[E], int t, const Acc& acc,
     for (int r = 0; r < R; ++r) a[r] = v[q + r * B];
     if constexpr (NS > 1) {
       constexpr int TS = N / (NS * R);
+      // byte step between this butterfly's twiddles (twiddle r at r * step).  The
+      // empty asm makes it opaque, so a kernel running several transforms recomputes
+      // the R-1 offsets per pass instead of keeping them live across the kernel.
+      unsigned step = (unsigned)(k * TS) * (unsigned)sizeof(cx<T>);
+      asm volatile("" : "+v"(step));
+      const char* twb = reinterpret_cast<const char*>(tw);
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        a[r] = cmul(a[r], tw[k * r * TS]);
+        a[r] = cmul(a[r], *reinterpret_cast<const cx<T>*>(twb + r * step));
       }
     }
     dft<T, R, INV>(a);

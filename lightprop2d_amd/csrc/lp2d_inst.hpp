@@ -13,17 +13,30 @@ constexpr int floor_pow2(int n) { return n < 2 ? 1 : 2 * floor_pow2(n >> 1); }
 constexpr int kLdsBudget = 160 * 1024 - 2048;
 
 // ----------------------------------------------------------------- kernels
-template <typename T, int N, int E, int LINES, bool INV, int PS>
+template <typename T, int N, int E, int LINES, bool INV, int PS, bool FEAT>
 __global__ __launch_bounds__((N / E) * LINES) void k_rows(PassArgs<T> a) {
   __shared__ cx<T> lds[LINES * PaddedLen<N, PS>::value];
   constexpr int TPF = (N + LINES - 1) / LINES;  // row tiles per field
   const int tile = blockIdx.x;
   const int b = tile / TPF;
   const int y0 = (tile - b * TPF) * LINES;
-  rows_body<T, N, E, LINES, INV, PS>(a, b, y0, cmin(LINES, N - y0), lds, threadIdx.x);
+  rows_body<T, N, E, LINES, INV, PS, FEAT>(a, b, y0, cmin(LINES, N - y0), lds, threadIdx.x);
 }
 
-template <typename T, int N, int E, int W, int MODE, int PS>
+template <typename T, int N, int E, int LINES, int PS>
+__global__ __launch_bounds__((N / E) * LINES) void k_rows_grad(PassArgs<T> a, int groups,
+                                                                float* partial) {
+  __shared__ cx<T> lds[LINES * PaddedLen<N, PS>::value];
+  constexpr int TPF = (N + LINES - 1) / LINES;
+  const int tile = blockIdx.x % TPF, g = blockIdx.x / TPF;
+  const int per = (a.batch + groups - 1) / groups;
+  const int b0 = g * per, b1 = cmin(a.batch, b0 + per);
+  const int y0 = tile * LINES;
+  rows_grad_body<T, N, E, LINES, PS>(a, b0, b1, y0, cmin(LINES, N - y0),
+                                     partial + (long long)g * N * N, lds, threadIdx.x);
+}
+
+template <typename T, int N, int E, int W, int MODE, int PS, int MIDK>
 __global__ __launch_bounds__((N / E) * W) void k_cols(PassArgs<T> a) {
   __shared__ cx<T> lds[W * PaddedLen<N, PS>::value];
   constexpr int SPF = N / W;  // column strips per field
@@ -33,7 +46,7 @@ __global__ __launch_bounds__((N / E) * W) void k_cols(PassArgs<T> a) {
   const int blk = (g % 8 == 0) ? (i % 8) * (g / 8) + i / 8 : i;
   const int b = blk / SPF;
   const int x0 = (blk - b * SPF) * W;
-  cols_body<T, N, E, W, MODE, PS>(a, b, x0, lds, threadIdx.x);
+  cols_body<T, N, E, W, MODE, PS, MIDK>(a, b, x0, lds, threadIdx.x);
 }
 
 template <typename T, int N, int E, int PS>
@@ -78,10 +91,26 @@ hipError_t launch_rows(const PassArgs<T>& a, bool inv, hipStream_t s) {
   constexpr int threads = (N / E) * LINES;
   constexpr int TPF = (N + LINES - 1) / LINES;
   const unsigned grid = (unsigned)((long long)a.batch * TPF);
-  if (inv)
-    k_rows<T, N, E, LINES, true, PS><<<grid, threads, 0, s>>>(a);
-  else
-    k_rows<T, N, E, LINES, false, PS><<<grid, threads, 0, s>>>(a);
+  const bool feat = (a.flags & (F_SPATIAL | F_GRAD | F_INTENSITY | F_NO_STORE)) != 0;
+  if (inv) {
+    if (feat)
+      k_rows<T, N, E, LINES, true, PS, true><<<grid, threads, 0, s>>>(a);
+    else
+      k_rows<T, N, E, LINES, true, PS, false><<<grid, threads, 0, s>>>(a);
+  } else {
+    if (feat)
+      k_rows<T, N, E, LINES, false, PS, true><<<grid, threads, 0, s>>>(a);
+    else
+      k_rows<T, N, E, LINES, false, PS, false><<<grid, threads, 0, s>>>(a);
+  }
+  return hipGetLastError();
+}
+
+template <typename T, int N, int E, int LINES, int PS>
+hipError_t launch_rows_grad(const PassArgs<T>& a, int groups, float* partial, hipStream_t s) {
+  constexpr int threads = (N / E) * LINES;
+  constexpr int TPF = (N + LINES - 1) / LINES;
+  k_rows_grad<T, N, E, LINES, PS><<<(unsigned)(TPF * groups), threads, 0, s>>>(a, groups, partial);
   return hipGetLastError();
 }
 
@@ -89,15 +118,31 @@ template <typename T, int N, int E, int W, int PS>
 hipError_t launch_cols(const PassArgs<T>& a, int mode, hipStream_t s) {
   constexpr int threads = (N / E) * W;
   const unsigned grid = (unsigned)((long long)a.batch * (N / W));
-  switch (mode) {
-    case MODE_FWD:
-      k_cols<T, N, E, W, MODE_FWD, PS><<<grid, threads, 0, s>>>(a);
+  // spectral stage class: none / transfer function + aperture + store / with masks
+  const int midk = (a.flags & (F_SMASK0 | F_SMASK1)) ? 2 : (a.flags & F_MID) ? 1 : 0;
+  constexpr int FI = MODE_FWD | MODE_INV;
+  switch (mode * 4 + midk) {
+    case MODE_FWD * 4 + 0:
+      k_cols<T, N, E, W, MODE_FWD, PS, 0><<<grid, threads, 0, s>>>(a);
       break;
-    case MODE_INV:
-      k_cols<T, N, E, W, MODE_INV, PS><<<grid, threads, 0, s>>>(a);
+    case MODE_INV * 4 + 0:
+      k_cols<T, N, E, W, MODE_INV, PS, 0><<<grid, threads, 0, s>>>(a);
       break;
-    default:
-      k_cols<T, N, E, W, MODE_FWD | MODE_INV, PS><<<grid, threads, 0, s>>>(a);
+    case MODE_INV * 4 + 1:
+      k_cols<T, N, E, W, MODE_INV, PS, 1><<<grid, threads, 0, s>>>(a);
+      break;
+    case FI * 4 + 0:
+    case FI * 4 + 1:
+      k_cols<T, N, E, W, FI, PS, 1><<<grid, threads, 0, s>>>(a);
+      break;
+    case FI * 4 + 2:
+      k_cols<T, N, E, W, FI, PS, 2><<<grid, threads, 0, s>>>(a);
+      break;
+    default:  // forward-only passes with a spectral stage, and inverse ones with masks
+      if (mode == MODE_FWD)
+        k_cols<T, N, E, W, MODE_FWD, PS, 2><<<grid, threads, 0, s>>>(a);
+      else
+        k_cols<T, N, E, W, MODE_INV, PS, 2><<<grid, threads, 0, s>>>(a);
       break;
   }
   return hipGetLastError();
@@ -116,8 +161,9 @@ void add_row(Registry<T>& r) {
   constexpr int lg = ilog2c(N);
   int& k = r.nrows[lg];
   if (k >= kMaxVariants) return;
-  r.rows[lg][k++] = RowVar<T>{&launch_rows<T, N, E, S::LINES, S::PS>, E, S::LINES, S::THREADS,
-                              S::LDS};
+  r.rows[lg][k++] = RowVar<T>{&launch_rows<T, N, E, S::LINES, S::PS>,
+                              &launch_rows_grad<T, N, E, S::LINES, S::PS>, E, S::LINES,
+                              S::THREADS, S::LDS};
 }
 
 template <typename T, int N, int E, int WT>

@@ -32,6 +32,7 @@ enum : int {
   F_SCALE = 1 << 10,     // epilogue: multiply by scale
   F_INTENSITY = 1 << 11, //           write |v|^2
   F_GRAD = 1 << 12,      //           adjoint phase-mask gradient
+  F_NO_STORE = 1 << 13,  //           skip the dst store (gradient-only epilogue)
 };
 constexpr int F_SPATIAL = F_MASK0 | F_MASK1 | F_LENS | F_PHASE | F_FIELD_OUT;
 constexpr int F_MID = F_H | F_APERTURE | F_SMASK0 | F_SMASK1 | F_SPEC_OUT;
@@ -170,7 +171,7 @@ __device__ __forceinline__ cx<T> spatial_pro(const PassArgs<T>& a, cx<T> v, int 
 // H = exp(-2*pi*i*frac(Kz*z)) exactly as beam2d.py:485-489 (the integer cycles of
 // Kz*z are dropped in FP64); Kz^2 <= 0 is evanescent: Kz := 0 (beam2d.py:307-308) so
 // H = 1 unless the caller opted into EV_ZERO / EV_DECAY.
-template <typename T>
+template <typename T, bool MASKS = true>
 __device__ __forceinline__ cx<T> spectral_mid(const PassArgs<T>& a, cx<T> v, int b, int ky,
                                               int kx, int n) {
   const int f = a.flags;
@@ -181,24 +182,54 @@ __device__ __forceinline__ cx<T> spectral_mid(const PassArgs<T>& a, cx<T> v, int
     const double fx = (double)sx * a.inv_ndL;
     const double fy = (double)sy * a.inv_ndL;
     if (f & F_H) {
+      // branch-free per element (divergent branches in the unrolled loop cost
+      // registers); only the uniform evanescent-mode test remains a branch
       const double kz2 = (a.k0sq - fx * fx) - fy * fy;
-      if (kz2 > 0.0) {
-        const double d = sqrt(kz2) * a.z;
-        const double fr = d - trunc(d);
-        v = cmul(v, exp_i_revolutions<T>(-fr));
-      } else if (kz2 < 0.0 && a.evan == EV_ZERO) {
-        v = mk((T)0, (T)0);
-      } else if (kz2 < 0.0 && a.evan == EV_DECAY) {
-        const T s = (T)exp(-2.0 * 3.14159265358979323846 * sqrt(-kz2) * fabs(a.z));
-        v = mk(v.re * s, v.im * s);
+      const double kz = sqrt(kz2 > 0.0 ? kz2 : 0.0);  // nan_to_num(sqrt(<0)) = 0 -> H = 1
+      const double d = kz * a.z;
+      const double fr = d - trunc(d);
+      cx<T> h = exp_i_revolutions<T>(-fr);
+      if (a.evan != EV_PASS) {
+        const T s = (a.evan == EV_ZERO)
+                        ? (T)0
+                        : (T)exp(-2.0 * 3.14159265358979323846 * sqrt(kz2 < 0.0 ? -kz2 : 0.0) *
+                                 fabs(a.z));
+        h = kz2 < 0.0 ? mk(s, (T)0) : h;
       }
+      v = cmul(v, h);
     }
     if (f & F_APERTURE) {
-      if (fx * fx + fy * fy > a.ap_r2) v = mk((T)0, (T)0);
+      const bool out = fx * fx + fy * fy > a.ap_r2;
+      v = mk(out ? (T)0 : v.re, out ? (T)0 : v.im);
     }
   }
-  if (f & F_SMASK0) v = mask_mul(a.s0, v, b, ky, kx);
-  if (f & F_SMASK1) v = mask_mul(a.s1, v, b, ky, kx);
+  if constexpr (MASKS) {
+    if (f & F_SMASK0) v = mask_mul(a.s0, v, b, ky, kx);
+    if (f & F_SMASK1) v = mask_mul(a.s1, v, b, ky, kx);
+  }
+  return v;
+}
+
+// v = g1, the adjoint of the propagation applied to the output gradient.  Undo the
+// spatial prologue of the forward call: returns grad_in = conj(phase factor) *
+// conj(lens factor) * g1.  The phase gradient is then Im(conj(u) * grad_in) with u
+// the forward input, which equals Im(conj(u * exp(i*phase) * lens) * g1).
+template <typename T>
+__device__ __forceinline__ cx<T> grad_fold(const PassArgs<T>& a, cx<T> v, int b, int y, int x,
+                                           int n) {
+  if (a.phase) {
+    double r = (double)a.phase[(long long)b * a.phase_sb + (long long)y * n + x] * kInv2Pi;
+    r -= rint(r);
+    v = cmulc(v, exp_i_revolutions<T>(r));
+  }
+  if (a.lens_rev != 0.0) {
+#pragma clang fp contract(off)
+    const double X = (double)(x - n / 2) * a.dL;
+    const double Y = (double)(y - n / 2) * a.dL;
+    double r = (X * X + Y * Y) * a.lens_rev;
+    r -= rint(r);
+    v = cmulc(v, exp_i_revolutions<T>(r));
+  }
   return v;
 }
 
@@ -210,23 +241,7 @@ __device__ __forceinline__ cx<T> spatial_epi(const PassArgs<T>& a, cx<T> v, int 
   const int f = a.flags;
   if (f & F_SCALE) v = mk(v.re * a.scale, v.im * a.scale);
   if (f & F_GRAD) {
-    // v = g1, the adjoint of the propagation applied to the output gradient.  Undo
-    // the spatial prologue of the forward call: grad_in = conj(phase factor) *
-    // conj(lens factor) * g1; grad_phase = Im(conj(u) * grad_in) with u the forward
-    // input, which equals Im(conj(u * exp(i*phase)) * g1).
-    if (a.phase) {
-      double r = (double)a.phase[(long long)b * a.phase_sb + (long long)y * n + x] * kInv2Pi;
-      r -= rint(r);
-      v = cmulc(v, exp_i_revolutions<T>(r));
-    }
-    if (a.lens_rev != 0.0) {
-#pragma clang fp contract(off)
-      const double X = (double)(x - n / 2) * a.dL;
-      const double Y = (double)(y - n / 2) * a.dL;
-      double r = (X * X + Y * Y) * a.lens_rev;
-      r -= rint(r);
-      v = cmulc(v, exp_i_revolutions<T>(r));
-    }
+    v = grad_fold(a, v, b, y, x, n);
     if (a.grad_phase) {
       const cx<T> u = a.u_saved[(long long)b * a.u_sb + (long long)y * n + x];
       a.grad_phase[goff] = (float)(u.re * v.im - u.im * v.re);
@@ -237,47 +252,108 @@ __device__ __forceinline__ cx<T> spatial_epi(const PassArgs<T>& a, cx<T> v, int 
 }
 
 // ----------------------------------------------------------------- row tiles
+// Clamp the row of an idle thread (past the end of a partial tile) onto a real row so
+// that its loads stay in bounds; its results are never stored.
+__device__ __forceinline__ int valid_row(int y, int ylast) { return y <= ylast ? y : ylast; }
+
 // LINES consecutive rows of field b starting at row y0.  Threads: (N/E) * LINES, the
 // thread index inside a row is the fast one, so each load/store instruction of a wave
 // covers contiguous memory.  lds: LINES * PaddedLen<N,PS> elements.
-template <typename T, int N, int E, int LINES, bool INV, int PS>
+// FEAT = false compiles the plain pass (load, FFT, optional scale, store): no
+// prologue/epilogue code, so its register budget is the FFT's alone.
+template <typename T, int N, int E, int LINES, bool INV, int PS, bool FEAT = true>
 __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, int nrows,
                                           cx<T>* lds, int tid) {
   constexpr int TL = N / E;
   constexpr int L = PaddedLen<N, PS>::value;
   const int l = tid / TL, t = tid - (tid / TL) * TL;
-  const int y = y0 + l;
+  const int y = valid_row(y0 + l, y0 + nrows - 1);
   const bool valid = l < nrows;
   const long long frow = (long long)b * a.fstride + (long long)y * N;
+  // uniform base (SGPRs) + one 32-bit per-lane offset
+  const long long ubase = (long long)b * a.fstride + (long long)y0 * N;
+  const unsigned lo = (unsigned)((y - y0) * N + t);
   cx<T> v[E];
-  const cx<T>* __restrict__ src = a.src + frow;
+  const cx<T>* __restrict__ src = a.src + ubase;
 #pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const int x = t + TL * m;
-    v[m] = valid ? src[x] : mk((T)0, (T)0);
-  }
-  if (a.flags & F_SPATIAL) {
+  for (int m = 0; m < E; ++m) v[m] = src[lo + (unsigned)(TL * m)];
+  if constexpr (FEAT) {
+    if (a.flags & F_SPATIAL) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const int x = t + TL * m;
-      if (valid) {
-        v[m] = spatial_pro(a, v[m], b, y, x, N);
-        if (a.flags & F_FIELD_OUT) a.field_out[frow + x] = v[m];
+      for (int m = 0; m < E; ++m) {
+        const int x = t + TL * m;
+        if (valid) {
+          v[m] = spatial_pro(a, v[m], b, y, x, N);
+          if (a.flags & F_FIELD_OUT) a.field_out[frow + x] = v[m];
+        }
       }
     }
   }
   LineAcc<T, PS> acc{lds + l * L};
   line_fft<T, N, E, INV>(v, t, acc, (INV ? a.tw_inv : a.tw) + (N - 2));
-  cx<T>* __restrict__ dst = a.dst + frow;
+  cx<T>* __restrict__ dst = a.dst + ubase;
+  if constexpr (FEAT) {
+    const bool store = (a.flags & F_NO_STORE) == 0;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int x = t + TL * m;
+      if (valid) {
+        const cx<T> o = (a.flags & (F_SCALE | F_GRAD | F_INTENSITY))
+                            ? spatial_epi(a, v[m], b, y, x, N, frow + x)
+                            : v[m];
+        if (store) dst[lo + (unsigned)(TL * m)] = o;
+      }
+    }
+  } else {
+    const T sc = (a.flags & F_SCALE) ? a.scale : (T)1;
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+      if (valid) dst[lo + (unsigned)(TL * m)] = mk(v[m].re * sc, v[m].im * sc);
+  }
+}
+
+// Last row pass of the adjoint for fields [b0, b1) that share one phase screen: the
+// inverse row FFT and gradient fold per field, with the phase gradient of every pixel
+// summed over the fields in registers and written once to `partial` (n x n).  The
+// screen factor conj(exp(i*phase) * lens) is the same for every field, so it is
+// computed once per pixel before the field loop.
+template <typename T, int N, int E, int LINES, int PS>
+__device__ __forceinline__ void rows_grad_body(const PassArgs<T>& a, int b0, int b1, int y0,
+                                               int nrows, float* __restrict__ partial,
+                                               cx<T>* lds, int tid) {
+  constexpr int TL = N / E;
+  constexpr int L = PaddedLen<N, PS>::value;
+  const int l = tid / TL, t = tid - (tid / TL) * TL;
+  const int y = valid_row(y0 + l, y0 + nrows - 1);
+  const bool valid = l < nrows;
+  const bool store = (a.flags & F_NO_STORE) == 0;
+  LineAcc<T, PS> acc{lds + l * L};
+  cx<T> ec[E];
+  float gsum[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
-    const int x = t + TL * m;
-    if (valid) {
-      const cx<T> o = (a.flags & (F_SCALE | F_GRAD | F_INTENSITY))
-                          ? spatial_epi(a, v[m], b, y, x, N, frow + x)
-                          : v[m];
-      dst[x] = o;
+    ec[m] = grad_fold(a, mk(a.scale, (T)0), 0, y, t + TL * m, N);
+    gsum[m] = 0.f;
+  }
+  for (int b = b0; b < b1; ++b) {
+    const long long frow = (long long)b * a.fstride + (long long)y * N;
+    cx<T> v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = a.src[frow + t + TL * m];
+    line_fft<T, N, E, true>(v, t, acc, a.tw_inv + (N - 2));
+    const cx<T>* __restrict__ us = a.u_saved + (long long)b * a.u_sb + (long long)y * N;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int x = t + TL * m;
+      const cx<T> g = cmul(v[m], ec[m]);
+      const cx<T> u = us[x];
+      gsum[m] += (float)(u.re * g.im - u.im * g.re);
+      if (store && valid) a.dst[frow + x] = g;
     }
+  }
+  if (valid) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) partial[(long long)y * N + t + TL * m] = gsum[m];
   }
 }
 
@@ -285,35 +361,45 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
 // W adjacent columns [x0, x0+W) of field b.  Threads: (N/E) * W with the column index
 // fastest, so every load/store instruction reads W*sizeof(cx) contiguous bytes from
 // each of 64/W rows.  lds: W * PaddedLen<N,PS> elements, interleaved by column.
-template <typename T, int N, int E, int W, int MODE, int PS>
+// MIDK selects the spectral stage compiled in: 0 none, 1 transfer function +
+// aperture + spectrum store, 2 everything (adds the mask multipliers).
+template <typename T, int N, int E, int W, int MODE, int PS, int MIDK = 2>
 __device__ __forceinline__ void cols_body(const PassArgs<T>& a, int b, int x0, cx<T>* lds,
                                           int tid) {
   constexpr int TL = N / E;
   const int c = tid % W, t = tid / W;
   const int x = x0 + c;
-  const long long fbase = (long long)b * a.fstride + x;
+  // uniform base (SGPRs) + one 32-bit per-lane offset: every load/store of the strip
+  // addresses as base + lane offset + compile-time constant
+  const long long ubase = (long long)b * a.fstride + x0;
+  const unsigned lo = (unsigned)(t * N + c);
   cx<T> v[E];
-  const cx<T>* __restrict__ src = a.src + fbase;
+  const cx<T>* __restrict__ src = a.src + ubase;
 #pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = src[(long long)(t + TL * m) * N];
+  for (int m = 0; m < E; ++m) v[m] = src[lo + (unsigned)(TL * N * m)];
   InterAcc<T, PS, W> acc{lds + c};
   if constexpr ((MODE & MODE_FWD) != 0) line_fft<T, N, E, false>(v, t, acc, a.tw + (N - 2));
-  if (a.flags & F_MID) {
+  if constexpr (MIDK > 0) {
+    if (a.flags & F_MID) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const int ky = t + TL * m;
-      v[m] = spectral_mid(a, v[m], b, ky, x, N);
-      if (a.flags & F_SPEC_OUT) a.spec_out[fbase + (long long)ky * N] = v[m];
+      for (int m = 0; m < E; ++m) {
+        const int ky = t + TL * m;
+        v[m] = spectral_mid<T, MIDK == 2>(a, v[m], b, ky, x, N);
+        if (a.flags & F_SPEC_OUT) a.spec_out[ubase + (lo + (unsigned)(TL * N * m))] = v[m];
+      }
     }
   }
+  // Keep the inverse transform's twiddle loads below the spectral stage: hoisted above
+  // it they stay live across it and cost tens of registers.
+  asm volatile("" ::: "memory");
   if constexpr ((MODE & MODE_INV) != 0) line_fft<T, N, E, true>(v, t, acc, a.tw_inv + (N - 2));
-  cx<T>* __restrict__ dst = a.dst + fbase;
+  cx<T>* __restrict__ dst = a.dst + ubase;
   const bool sc = (a.flags & F_SCALE) != 0;
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     cx<T> o = v[m];
     if (sc) o = mk(o.re * a.scale, o.im * a.scale);
-    dst[(long long)(t + TL * m) * N] = o;
+    dst[lo + (unsigned)(TL * N * m)] = o;
   }
 }
 
@@ -394,7 +480,7 @@ __device__ __forceinline__ void small_body(const PassArgs<T>& a, int b, int mode
       const cx<T> o = (a.flags & (F_SCALE | F_GRAD | F_INTENSITY))
                           ? spatial_epi(a, v[m], b, y, x, N, fb + (long long)y * N + x)
                           : v[m];
-      dst[x] = o;
+      if ((a.flags & F_NO_STORE) == 0) dst[x] = o;
     }
   }
 }

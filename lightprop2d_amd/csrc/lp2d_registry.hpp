@@ -16,8 +16,13 @@ template <typename T>
 using SmallLaunch = hipError_t (*)(const PassArgs<T>&, int mode, hipStream_t);
 
 template <typename T>
+using RowGradLaunch = hipError_t (*)(const PassArgs<T>&, int groups, float* partial,
+                                     hipStream_t);
+
+template <typename T>
 struct RowVar {
   RowLaunch<T> fn;
+  RowGradLaunch<T> grad_fn;  // adjoint last pass with the batch-summed phase gradient
   int E, lines, threads, lds;
 };
 template <typename T>
@@ -45,8 +50,10 @@ struct Registry {
 };
 
 void register_f32_a(Registry<float>& r);   // n = 2 .. 256
-void register_f32_b(Registry<float>& r);   // n = 512 .. 8192
+void register_f32_b(Registry<float>& r);   // n = 512, 1024
+void register_f32_c(Registry<float>& r);   // n = 2048 .. 8192
 void register_f64_a(Registry<double>& r);  // n = 2 .. 256
-void register_f64_b(Registry<double>& r);  // n = 512 .. 8192
+void register_f64_b(Registry<double>& r);  // n = 512, 1024
+void register_f64_c(Registry<double>& r);  // n = 2048 .. 8192
 
 }  // namespace lp2d

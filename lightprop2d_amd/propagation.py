@@ -228,11 +228,13 @@ def propagate_adjoint(grad_out: torch.Tensor, field_in: Optional[torch.Tensor], 
                       area_size: float, wl: float, phase: Optional[torch.Tensor] = None,
                       aperture_radius: float = 0.0, lens_f: float = 0.0,
                       evanescent: str = "reference", want_phase_grad: bool = True,
-                      out: Optional[torch.Tensor] = None):
+                      want_field_grad: bool = True, out: Optional[torch.Tensor] = None):
     """Adjoint of `propagate` (vector-Jacobian product, PyTorch complex convention).
 
-    Returns (grad_field, grad_phase) where grad_phase has the phase's shape (summed
-    over the batch for a shared (n, n) screen) or is None."""
+    Returns (grad_field, grad_phase).  grad_phase has the phase's shape: per field for a
+    (batch, n, n) phase, summed over the batch (inside the kernels) for a shared (n, n)
+    screen; None when not requested.  grad_field is None when want_field_grad is False
+    (its buffer is then only the passes' workspace)."""
     lib = _lib()
     n, batch, dt = _field_geom(grad_out, "grad_out")
     if out is None:
@@ -245,19 +247,14 @@ def propagate_adjoint(grad_out: torch.Tensor, field_in: Optional[torch.Tensor], 
         _field_geom(field_in, "field_in")
     o = optics(area_size, wl, z=z, aperture_radius=aperture_radius, lens_f=lens_f,
                evanescent=evanescent)
+    flags = 0 if want_field_grad else L.LP2D_ADJ_NO_GRAD_IN
     with torch.cuda.device(grad_out.device):
         L.check(lib.lp2d_propagate_adjoint(n, batch, dt, _ptr(grad_out), _ptr(field_in),
                                            _ptr(ph), psb, C.byref(o), _ptr(out), _ptr(gph),
-                                           _stream(grad_out)), "propagate_adjoint")
-        if gph is not None:
-            if psb == 0:
-                if batch > 1:
-                    red = torch.empty((n, n), dtype=torch.float32, device=grad_out.device)
-                    sum_planes(gph, red)
-                    gph = red
-                else:
-                    gph = gph.view(n, n)
-    return out, gph
+                                           flags, _stream(grad_out)), "propagate_adjoint")
+    if gph is not None and psb == 0:
+        gph = gph[0]
+    return (out if want_field_grad else None), gph
 
 
 def sum_planes(planes: torch.Tensor, out: torch.Tensor):

@@ -64,7 +64,9 @@ def test_argument_errors_before_any_device_work(lib):
     bad = _lib.lp2d_optics(-1.0, 1e-4, 0.1, 0.0, 0.0, 0, 0)
     assert lib.lp2d_propagate(64, 1, 0, p, p, None, 0, C.byref(bad), None, None) == _lib.LP2D_E_ARG
     assert lib.lp2d_propagate(64, 1, 0, p, p, None, 0, C.byref(o), p, None) == _lib.LP2D_E_ARG
-    assert lib.lp2d_propagate_adjoint(64, 1, 0, p, None, p, 0, C.byref(o), p, None,
+    assert lib.lp2d_propagate_adjoint(64, 1, 0, p, None, p, 0, C.byref(o), p, None, 0,
+                                      None) == _lib.LP2D_E_ARG
+    assert lib.lp2d_propagate_adjoint(64, 1, 0, p, p, None, 0, C.byref(o), C.c_void_p(32), None, 6,
                                       None) == _lib.LP2D_E_ARG
     m = (_lib.lp2d_mask * 1)(_lib.lp2d_mask(16, 42, 0, 0, 0, 0))
     assert lib.lp2d_spatial_apply(64, 1, 0, p, m, 1, None, None, 0, None, None) == _lib.LP2D_E_DTYPE

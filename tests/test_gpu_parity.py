@@ -193,18 +193,29 @@ def test_adjoint_dot_product(n):
     assert abs(lhs - rhs) < 1e-10 * abs(lhs)
 
 
-def test_adjoint_shared_phase_batch_reduction():
+@pytest.mark.parametrize("n,batch", [(128, 3), (512, 5), (1024, 12)])
+def test_adjoint_shared_phase_batch_reduction(n, batch):
+    """Shared screen: the phase gradient is summed over the batch inside the kernels
+    (fused into the last row pass for n >= 256); grad_in can be skipped."""
     from lightprop2d_amd import propagation as P
-    n, area, wl, z = 128, 200e-4, 532e-7, 0.01
+    area, wl, z = 200e-4, 532e-7, 0.01
     rs = np.random.RandomState(9)
-    u = cplx(rs, (3, n, n), np.complex64)
-    g = cplx(rs, (3, n, n), np.complex64)
+    u = cplx(rs, (batch, n, n), np.complex64)
+    g = cplx(rs, (batch, n, n), np.complex64)
     phi = rs.uniform(0, 2 * np.pi, (n, n)).astype(np.float32)
     gu, gp = P.propagate_adjoint(to_dev(g), to_dev(u), z, area, wl, phase=to_dev(phi))
-    exp = sum(oracle.propagate_field_adjoint(g[i], u[i], area, wl, z,
-                                             phase_mask=phi.astype(np.float64))[1] for i in range(3))
+    exp_u, exp_p = [], 0
+    for i in range(batch):
+        a, b = oracle.propagate_field_adjoint(g[i], u[i], area, wl, z, phase_mask=phi.astype(np.float64))
+        exp_u.append(a)
+        exp_p = exp_p + b
     assert gp.shape == (n, n)
-    assert rel_l2(host(gp), exp) < TOL64
+    assert rel_l2(host(gp), exp_p) < TOL64
+    assert rel_l2(host(gu), np.stack(exp_u)) < TOL64
+    gu2, gp2 = P.propagate_adjoint(to_dev(g), to_dev(u), z, area, wl, phase=to_dev(phi),
+                                   want_field_grad=False)
+    assert gu2 is None
+    assert rel_l2(host(gp2), host(gp)) < 1e-6
 
 
 # ------------------------------------------------------------------ grids / intensity
