@@ -63,6 +63,21 @@ def cpu_baseline(n, budget_s):
                       f"in {el:.1f} s", "omp_num_threads": threads}
 
 
+def profiled_traffic(kernel_name):
+    """HBM bytes per launch of `kernel_name` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (profiles/hbm_traffic_propagate_1024_c64.md), or None."""
+    p = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                     "hbm_traffic_propagate_1024_c64.md")
+    try:
+        kv = dict(l.split(": ", 1) for l in open(p).read().splitlines() if ": " in l
+                  and l.split(": ", 1)[0] in ("bench_kernel", "traffic_bytes_per_launch"))
+    except OSError:
+        return None
+    if kv.get("bench_kernel") != kernel_name:
+        return None
+    return int(kv["traffic_bytes_per_launch"])
+
+
 def trace_stats(lib, L):
     import ctypes as C
     arr = (L.lp2d_trace_stat * 64)()
@@ -146,8 +161,10 @@ def main():
     alg_bytes = 16.0 * n * n * B
     achieved = alg_bytes / avg_s / 1e9
     e2e = 16.0 * n * n * value / 1e9
+    traffic = profiled_traffic(dom_name) if B == 32 else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": "profiles/hbm_traffic_propagate_1024_c64.md" if traffic else None,
                 "kernel": dom_name, "kernel_avg_us": round(avg_s * 1e6, 2),
                 "alg_bytes_per_launch": int(alg_bytes),
                 "end_to_end_GBs": round(e2e, 1),
