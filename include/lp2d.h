@@ -200,6 +200,11 @@ int lp2d_trace_collect(lp2d_trace_stat* stats, int max_stats, int* n_stats);
 int lp2d_variant_count(int n, int dtype, int kind);
 int lp2d_set_variant(int n, int dtype, int kind, int variant);
 int lp2d_get_variant(int n, int dtype, int kind);
+/* Fields per chunk of the three-sweep field-to-field propagation at size n (0, the
+ * default, runs the whole batch per sweep).  A chunk's intermediate planes stay in
+ * the Infinity Cache between its sweeps.  Used by the autotuner. */
+int lp2d_set_chunk(int n, int dtype, int fields);
+int lp2d_get_chunk(int n, int dtype);
 /* Describes a variant as "E=16 W=16 ..." into buf (returns length). */
 int lp2d_variant_name(int n, int dtype, int kind, int variant, char* buf, int buflen);
 

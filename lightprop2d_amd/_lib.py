@@ -39,7 +39,7 @@ EXPORTED = [
     "lp2d_propagate", "lp2d_propagate_adjoint", "lp2d_sum_planes", "lp2d_batch_create",
     "lp2d_batch_run", "lp2d_batch_destroy", "lp2d_batch_set_grid", "lp2d_variant_count",
     "lp2d_set_variant", "lp2d_get_variant", "lp2d_variant_name", "lp2d_trace_enable",
-    "lp2d_trace_collect", "lp2d_moments",
+    "lp2d_trace_collect", "lp2d_moments", "lp2d_set_chunk", "lp2d_get_chunk",
 ]
 
 
@@ -106,6 +106,8 @@ _SIGS = {
     "lp2d_variant_name": (_I, [_I, _I, _I, _I, C.c_char_p, _I]),
     "lp2d_trace_enable": (_I, [_I]),
     "lp2d_moments": (_I, [_I, _I, _I, _P, C.c_double, _P, _P]),
+    "lp2d_set_chunk": (_I, [_I, _I, _I]),
+    "lp2d_get_chunk": (_I, [_I, _I]),
     "lp2d_trace_collect": (_I, [C.POINTER(lp2d_trace_stat), _I, C.POINTER(_I)]),
 }
 

@@ -44,6 +44,7 @@ DeviceTables g_dev[kMaxDevices];
 Registry<float>* g_rf = nullptr;
 Registry<double>* g_rd = nullptr;
 int g_sel[2][2][kLogMax + 1];  // [dtype][kind][log2 n] -> selected variant
+int g_chunk[2][kLogMax + 1];   // fields per three-sweep chunk (0 = whole batch)
 
 void ensure_registry() {
   static std::once_flag once;
@@ -59,6 +60,7 @@ void ensure_registry() {
     register_f64_b(*g_rd);
     register_f64_c(*g_rd);
     std::memset(g_sel, 0, sizeof(g_sel));
+    std::memset(g_chunk, 0, sizeof(g_chunk));
   });
 }
 
@@ -540,8 +542,33 @@ int op_spectral(int n, int lg, int batch, void* spec, const lp2d_mask* masks, in
 }
 
 template <typename T>
+int op_propagate_chunk(int n, int lg, int batch, const void* in, void* out, const float* phase,
+                       int64_t phase_sb, const lp2d_optics* o, void* spec, hipStream_t s);
+
+// Field-to-field propagation.  Large sizes run three sweeps per chunk of fields so
+// that a chunk's intermediate planes are still in the Infinity Cache when the next
+// sweep reads them (chunk size from lp2d_set_chunk; 0 = the whole batch at once).
+template <typename T>
 int op_propagate(int n, int lg, int batch, const void* in, void* out, const float* phase,
                  int64_t phase_sb, const lp2d_optics* o, void* spec, hipStream_t s) {
+  const int chunk = g_chunk[dt<T>()][lg];
+  if (chunk <= 0 || chunk >= batch || use_small<T>(lg))
+    return op_propagate_chunk<T>(n, lg, batch, in, out, phase, phase_sb, o, spec, s);
+  const long long fs = (long long)n * n;
+  for (int c0 = 0; c0 < batch; c0 += chunk) {
+    const int cb = batch - c0 < chunk ? batch - c0 : chunk;
+    if (int rc = op_propagate_chunk<T>(
+            n, lg, cb, (const cx<T>*)in + c0 * fs, (cx<T>*)out + c0 * fs,
+            phase ? phase + c0 * phase_sb : nullptr, phase_sb, o,
+            spec ? (void*)((cx<T>*)spec + c0 * fs) : nullptr, s))
+      return rc;
+  }
+  return LP2D_OK;
+}
+
+template <typename T>
+int op_propagate_chunk(int n, int lg, int batch, const void* in, void* out, const float* phase,
+                       int64_t phase_sb, const lp2d_optics* o, void* spec, hipStream_t s) {
   DeviceTables* t;
   if (int rc = current_tables(&t)) return rc;
   PassArgs<T> a = base_args<T>(t, n, batch);
@@ -830,6 +857,22 @@ int lp2d_set_variant(int n, int dtype, int kind, int variant) {
   return LP2D_OK;
 }
 
+int lp2d_set_chunk(int n, int dtype, int fields) {
+  ensure_registry();
+  const int lg = log2_exact(n);
+  if (lg < 1 || lg > kLogMax || (dtype != 0 && dtype != 1)) return fail(LP2D_E_SIZE, "bad n or dtype");
+  if (fields < 0) return fail(LP2D_E_ARG, "fields must be >= 0");
+  g_chunk[dtype][lg] = fields;
+  return LP2D_OK;
+}
+
+int lp2d_get_chunk(int n, int dtype) {
+  ensure_registry();
+  const int lg = log2_exact(n);
+  if (lg < 1 || lg > kLogMax || (dtype != 0 && dtype != 1)) return fail(LP2D_E_SIZE, "bad n or dtype");
+  return g_chunk[dtype][lg];
+}
+
 int lp2d_get_variant(int n, int dtype, int kind) {
   if ((kind != 0 && kind != 1) || lp2d_variant_count(n, dtype, kind) == 0)
     return fail(LP2D_E_SIZE, "no variants");
@@ -852,7 +895,9 @@ int lp2d_variant_name(int n, int dtype, int kind, int variant, char* buf, int bu
   const int W = dtype ? g_rd->cols[lg][variant].W : g_rf->cols[lg][variant].W;
   const int th = dtype ? g_rd->cols[lg][variant].threads : g_rf->cols[lg][variant].threads;
   const int lds = dtype ? g_rd->cols[lg][variant].lds : g_rf->cols[lg][variant].lds;
-  return snprintf(buf, buflen, "cols E=%d W=%d threads=%d lds=%d", E, W, th, lds);
+  const bool hoist = dtype ? g_rd->cols[lg][variant].hoist : g_rf->cols[lg][variant].hoist;
+  return snprintf(buf, buflen, "cols E=%d W=%d threads=%d lds=%d%s", E, W, th, lds,
+                  hoist ? " shared-H" : "");
 }
 
 }  // extern "C"

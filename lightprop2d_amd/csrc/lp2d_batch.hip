@@ -109,7 +109,7 @@ __device__ __forceinline__ void col_tile(PassArgs<float>& a, int tile, int mid,
   using S = TileShape<N>;
   a.src = a.dst;
   a.flags = mid;
-  cols_body<float, N, kEb, S::LINES, MODE_FWD | MODE_INV, S::PS, 1>(a, 0, tile * S::LINES,
+  cols_body<float, N, kEb, S::LINES, MODE_FWD | MODE_INV, S::PS, 1>(a, 0, 1, tile * S::LINES,
                                                                    lds, threadIdx.x);
 }
 

@@ -36,17 +36,20 @@ __global__ __launch_bounds__((N / E) * LINES) void k_rows_grad(PassArgs<T> a, in
                                      partial + (long long)g * N * N, lds, threadIdx.x);
 }
 
+// Grid: (N/W strips) x (field groups of `per` fields).
 template <typename T, int N, int E, int W, int MODE, int PS, int MIDK>
-__global__ __launch_bounds__((N / E) * W) void k_cols(PassArgs<T> a) {
+__global__ __launch_bounds__((N / E) * W) void k_cols(PassArgs<T> a, int per) {
   __shared__ cx<T> lds[W * PaddedLen<N, PS>::value];
   constexpr int SPF = N / W;  // column strips per field
   const int g = gridDim.x, i = blockIdx.x;
   // Workgroups go to the 8 XCDs round robin.  Give each XCD a contiguous run of
   // strips so that narrow strips sharing 128-byte lines meet in the same L2.
   const int blk = (g % 8 == 0) ? (i % 8) * (g / 8) + i / 8 : i;
-  const int b = blk / SPF;
-  const int x0 = (blk - b * SPF) * W;
-  cols_body<T, N, E, W, MODE, PS, MIDK>(a, b, x0, lds, threadIdx.x);
+  const int grp = blk / SPF;
+  const int x0 = (blk - grp * SPF) * W;
+  const int b0 = grp * per;
+  const int b1 = cmin(a.batch, b0 + per);
+  cols_body<T, N, E, W, MODE, PS, MIDK>(a, b0, b1, x0, lds, threadIdx.x);
 }
 
 template <typename T, int N, int E, int PS>
@@ -114,35 +117,46 @@ hipError_t launch_rows_grad(const PassArgs<T>& a, int groups, float* partial, hi
   return hipGetLastError();
 }
 
-template <typename T, int N, int E, int W, int PS>
+// Fields per column workgroup: share the transfer function across a group of fields
+// while keeping at least ~kColBlocks workgroups in flight.
+constexpr int kColBlocks = 1024;
+
+template <typename T, int N, int E, int W, int PS, bool HOIST>
 hipError_t launch_cols(const PassArgs<T>& a, int mode, hipStream_t s) {
   constexpr int threads = (N / E) * W;
-  const unsigned grid = (unsigned)((long long)a.batch * (N / W));
+  constexpr int SPF = N / W;
+  constexpr int MH = HOIST ? 3 : 1;  // spectral-stage class of the H / aperture path
+  const bool shared_h = HOIST && (a.flags & F_MID) && !(a.flags & (F_SMASK0 | F_SMASK1));
+  int per = shared_h ? (int)((long long)a.batch * SPF / kColBlocks) : 1;
+  if (per < 1) per = 1;
+  if (per > 8) per = 8;
+  const int groups = (a.batch + per - 1) / per;
+  const unsigned grid = (unsigned)((long long)groups * SPF);
   // spectral stage class: none / transfer function + aperture + store / with masks
   const int midk = (a.flags & (F_SMASK0 | F_SMASK1)) ? 2 : (a.flags & F_MID) ? 1 : 0;
   constexpr int FI = MODE_FWD | MODE_INV;
   switch (mode * 4 + midk) {
     case MODE_FWD * 4 + 0:
-      k_cols<T, N, E, W, MODE_FWD, PS, 0><<<grid, threads, 0, s>>>(a);
+      k_cols<T, N, E, W, MODE_FWD, PS, 0><<<grid, threads, 0, s>>>(a, per);
       break;
     case MODE_INV * 4 + 0:
-      k_cols<T, N, E, W, MODE_INV, PS, 0><<<grid, threads, 0, s>>>(a);
+      k_cols<T, N, E, W, MODE_INV, PS, 0><<<grid, threads, 0, s>>>(a, per);
       break;
     case MODE_INV * 4 + 1:
-      k_cols<T, N, E, W, MODE_INV, PS, 1><<<grid, threads, 0, s>>>(a);
+      k_cols<T, N, E, W, MODE_INV, PS, MH><<<grid, threads, 0, s>>>(a, per);
       break;
     case FI * 4 + 0:
     case FI * 4 + 1:
-      k_cols<T, N, E, W, FI, PS, 1><<<grid, threads, 0, s>>>(a);
+      k_cols<T, N, E, W, FI, PS, MH><<<grid, threads, 0, s>>>(a, per);
       break;
     case FI * 4 + 2:
-      k_cols<T, N, E, W, FI, PS, 2><<<grid, threads, 0, s>>>(a);
+      k_cols<T, N, E, W, FI, PS, 2><<<grid, threads, 0, s>>>(a, per);
       break;
     default:  // forward-only passes with a spectral stage, and inverse ones with masks
       if (mode == MODE_FWD)
-        k_cols<T, N, E, W, MODE_FWD, PS, 2><<<grid, threads, 0, s>>>(a);
+        k_cols<T, N, E, W, MODE_FWD, PS, 2><<<grid, threads, 0, s>>>(a, per);
       else
-        k_cols<T, N, E, W, MODE_INV, PS, 2><<<grid, threads, 0, s>>>(a);
+        k_cols<T, N, E, W, MODE_INV, PS, 2><<<grid, threads, 0, s>>>(a, per);
       break;
   }
   return hipGetLastError();
@@ -166,13 +180,14 @@ void add_row(Registry<T>& r) {
                               S::THREADS, S::LDS};
 }
 
-template <typename T, int N, int E, int WT>
+template <typename T, int N, int E, int WT, bool HOIST = false>
 void add_col(Registry<T>& r) {
   using S = ColShape<T, N, E, WT>;
   constexpr int lg = ilog2c(N);
   int& k = r.ncols[lg];
   if (k >= kMaxVariants) return;
-  r.cols[lg][k++] = ColVar<T>{&launch_cols<T, N, E, S::W, S::PS>, E, S::W, S::THREADS, S::LDS};
+  r.cols[lg][k++] = ColVar<T>{&launch_cols<T, N, E, S::W, S::PS, HOIST>, E, S::W, S::THREADS,
+                              S::LDS, HOIST};
 }
 
 template <typename T, int N>
@@ -197,6 +212,8 @@ void register_n(Registry<T>& r) {
       add_row<T, N, 32>(r);
       add_col<T, N, 32, 16>(r);
       add_col<T, N, 32, 8>(r);
+      add_col<T, N, 32, 8, true>(r);
+      add_col<T, N, 16, 8, true>(r);
     } else {
       add_row<T, N, 16>(r);
       add_col<T, N, 16, 8>(r);

@@ -233,22 +233,125 @@ __device__ __forceinline__ cx<T> grad_fold(const PassArgs<T>& a, cx<T> v, int b,
   return v;
 }
 
-// Epilogue at spatial (y, x) of field b; element offset `off` inside the field.
-// Returns the value to store in dst.
-template <typename T>
-__device__ __forceinline__ cx<T> spatial_epi(const PassArgs<T>& a, cx<T> v, int b, int y, int x,
-                                             int n, long long goff) {
+// ----------------------------------------------------------------- array forms
+// The prologue and epilogue applied to the E elements x = t + TL*m of one line of a
+// thread.  Each feature is one uniform branch around a branch-free loop over the E
+// elements, so the loads of all elements can be issued together.
+template <typename T, int E>
+__device__ __forceinline__ void mask_mul_all(const MaskD& mk_, cx<T> (&v)[E], int b, int y,
+                                             int t, int TL) {
+  const long long base = (long long)b * mk_.sb + (long long)y * mk_.sy;
+  switch (mk_.dtype) {
+    case MK_U8:
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const T s = (T)(((const uint8_t*)mk_.ptr)[base + (long long)(t + TL * m) * mk_.sx] != 0);
+        v[m] = mk(v[m].re * s, v[m].im * s);
+      }
+      break;
+    case MK_F32:
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const T s = (T)((const float*)mk_.ptr)[base + (long long)(t + TL * m) * mk_.sx];
+        v[m] = mk(v[m].re * s, v[m].im * s);
+      }
+      break;
+    case MK_F64:
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const T s = (T)((const double*)mk_.ptr)[base + (long long)(t + TL * m) * mk_.sx];
+        v[m] = mk(v[m].re * s, v[m].im * s);
+      }
+      break;
+    case MK_C64:
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const cx<float> w = ((const cx<float>*)mk_.ptr)[base + (long long)(t + TL * m) * mk_.sx];
+        v[m] = cmul(v[m], mk((T)w.re, (T)w.im));
+      }
+      break;
+    default:
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const cx<double> w = ((const cx<double>*)mk_.ptr)[base + (long long)(t + TL * m) * mk_.sx];
+        v[m] = cmul(v[m], mk((T)w.re, (T)w.im));
+      }
+      break;
+  }
+}
+
+// exp(i * (X^2 + Y^2) * k0/(2f)) for the E pixels of the line (reference lens phase,
+// beam2d.py:508), conjugated when CONJ.
+template <typename T, int E, bool CONJ>
+__device__ __forceinline__ void lens_mul_all(const PassArgs<T>& a, cx<T> (&v)[E], int y, int t,
+                                             int TL, int n) {
+#pragma clang fp contract(off)
+  const double Y = (double)(y - n / 2) * a.dL;
+  const double Y2 = Y * Y;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const double X = (double)(t + TL * m - n / 2) * a.dL;
+    double r = (X * X + Y2) * a.lens_rev;
+    r -= rint(r);
+    const cx<T> e = exp_i_revolutions<T>(r);
+    v[m] = CONJ ? cmulc(v[m], e) : cmul(v[m], e);
+  }
+}
+
+// exp(i * phase[y, x]) (conjugated when CONJ) for the E pixels of the line.
+template <typename T, int E, bool CONJ>
+__device__ __forceinline__ void phase_mul_all(const PassArgs<T>& a, cx<T> (&v)[E], int b, int y,
+                                              int t, int TL, int n) {
+  const float* ph = a.phase + (long long)b * a.phase_sb + (long long)y * n + t;
+  float p[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) p[m] = ph[TL * m];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    double r = (double)p[m] * kInv2Pi;
+    r -= rint(r);
+    const cx<T> e = exp_i_revolutions<T>(r);
+    v[m] = CONJ ? cmulc(v[m], e) : cmul(v[m], e);
+  }
+}
+
+template <typename T, int E>
+__device__ __forceinline__ void spatial_pro_all(const PassArgs<T>& a, cx<T> (&v)[E], int b, int y,
+                                                int t, int TL, int n) {
   const int f = a.flags;
-  if (f & F_SCALE) v = mk(v.re * a.scale, v.im * a.scale);
+  if (f & F_MASK0) mask_mul_all<T, E>(a.m0, v, b, y, t, TL);
+  if (f & F_MASK1) mask_mul_all<T, E>(a.m1, v, b, y, t, TL);
+  if (f & F_LENS) lens_mul_all<T, E, false>(a, v, y, t, TL, n);
+  if (f & F_PHASE) phase_mul_all<T, E, false>(a, v, b, y, t, TL, n);
+}
+
+// Epilogue over the E pixels of one line (scale, gradient fold and phase gradient,
+// intensity); `off` = element offset of pixel x = t in field b.
+template <typename T, int E>
+__device__ __forceinline__ void spatial_epi_all(const PassArgs<T>& a, cx<T> (&v)[E], int b, int y,
+                                                int t, int TL, int n, long long off, bool valid) {
+  const int f = a.flags;
+  if (f & F_SCALE) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = mk(v[m].re * a.scale, v[m].im * a.scale);
+  }
   if (f & F_GRAD) {
-    v = grad_fold(a, v, b, y, x, n);
+    if (a.phase) phase_mul_all<T, E, true>(a, v, b, y, t, TL, n);
+    if (a.lens_rev != 0.0) lens_mul_all<T, E, true>(a, v, y, t, TL, n);
     if (a.grad_phase) {
-      const cx<T> u = a.u_saved[(long long)b * a.u_sb + (long long)y * n + x];
-      a.grad_phase[goff] = (float)(u.re * v.im - u.im * v.re);
+      const cx<T>* u = a.u_saved + (long long)b * a.u_sb + (long long)y * n + t;
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const cx<T> w = u[TL * m];
+        if (valid) a.grad_phase[off + TL * m] = (float)(w.re * v[m].im - w.im * v[m].re);
+      }
     }
   }
-  if (f & F_INTENSITY) a.intensity[goff] = v.re * v.re + v.im * v.im;
-  return v;
+  if (f & F_INTENSITY) {
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+      if (valid) a.intensity[off + TL * m] = v[m].re * v[m].re + v[m].im * v[m].im;
+  }
 }
 
 // ----------------------------------------------------------------- row tiles
@@ -279,13 +382,11 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
   for (int m = 0; m < E; ++m) v[m] = src[lo + (unsigned)(TL * m)];
   if constexpr (FEAT) {
     if (a.flags & F_SPATIAL) {
+      spatial_pro_all<T, E>(a, v, b, y, t, TL, N);
+      if (a.flags & F_FIELD_OUT) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const int x = t + TL * m;
-        if (valid) {
-          v[m] = spatial_pro(a, v[m], b, y, x, N);
-          if (a.flags & F_FIELD_OUT) a.field_out[frow + x] = v[m];
-        }
+        for (int m = 0; m < E; ++m)
+          if (valid) a.field_out[frow + t + TL * m] = v[m];
       }
     }
   }
@@ -293,16 +394,11 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
   line_fft<T, N, E, INV>(v, t, acc, (INV ? a.tw_inv : a.tw) + (N - 2));
   cx<T>* __restrict__ dst = a.dst + ubase;
   if constexpr (FEAT) {
-    const bool store = (a.flags & F_NO_STORE) == 0;
+    spatial_epi_all<T, E>(a, v, b, y, t, TL, N, frow + t, valid);
+    if ((a.flags & F_NO_STORE) == 0) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const int x = t + TL * m;
-      if (valid) {
-        const cx<T> o = (a.flags & (F_SCALE | F_GRAD | F_INTENSITY))
-                            ? spatial_epi(a, v[m], b, y, x, N, frow + x)
-                            : v[m];
-        if (store) dst[lo + (unsigned)(TL * m)] = o;
-      }
+      for (int m = 0; m < E; ++m)
+        if (valid) dst[lo + (unsigned)(TL * m)] = v[m];
     }
   } else {
     const T sc = (a.flags & F_SCALE) ? a.scale : (T)1;
@@ -362,44 +458,72 @@ __device__ __forceinline__ void rows_grad_body(const PassArgs<T>& a, int b0, int
 // fastest, so every load/store instruction reads W*sizeof(cx) contiguous bytes from
 // each of 64/W rows.  lds: W * PaddedLen<N,PS> elements, interleaved by column.
 // MIDK selects the spectral stage compiled in: 0 none, 1 transfer function +
-// aperture + spectrum store, 2 everything (adds the mask multipliers).
+// aperture + spectrum store, 2 everything (adds the mask multipliers), 3 as 1 with the
+// factor shared by a group of fields.
+// The workgroup runs fields [b0, b1) of the same strip in turn.  With MIDK == 3 the
+// spectral factor H(kx, ky) * aperture is the same for every field of the call, so it
+// is computed once per element before the field loop and kept in registers (2E more
+// live registers: a separate schedule for the autotuner to weigh).
 template <typename T, int N, int E, int W, int MODE, int PS, int MIDK = 2>
-__device__ __forceinline__ void cols_body(const PassArgs<T>& a, int b, int x0, cx<T>* lds,
-                                          int tid) {
+__device__ __forceinline__ void cols_body(const PassArgs<T>& a, int b0, int b1, int x0,
+                                          cx<T>* lds, int tid) {
   constexpr int TL = N / E;
   const int c = tid % W, t = tid / W;
   const int x = x0 + c;
-  // uniform base (SGPRs) + one 32-bit per-lane offset: every load/store of the strip
-  // addresses as base + lane offset + compile-time constant
-  const long long ubase = (long long)b * a.fstride + x0;
   const unsigned lo = (unsigned)(t * N + c);
-  cx<T> v[E];
-  const cx<T>* __restrict__ src = a.src + ubase;
-#pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = src[lo + (unsigned)(TL * N * m)];
   InterAcc<T, PS, W> acc{lds + c};
-  if constexpr ((MODE & MODE_FWD) != 0) line_fft<T, N, E, false>(v, t, acc, a.tw + (N - 2));
-  if constexpr (MIDK > 0) {
-    if (a.flags & F_MID) {
+  const bool mid = (a.flags & F_MID) != 0;
+  const bool spec_out = (a.flags & F_SPEC_OUT) != 0;
+  cx<T> h[MIDK == 3 ? E : 1];
+  if constexpr (MIDK == 3) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const int ky = t + TL * m;
-        v[m] = spectral_mid<T, MIDK == 2>(a, v[m], b, ky, x, N);
-        if (a.flags & F_SPEC_OUT) a.spec_out[ubase + (lo + (unsigned)(TL * N * m))] = v[m];
+    for (int m = 0; m < E; ++m)
+      h[m] = mid ? spectral_mid<T, false>(a, mk((T)1, (T)0), 0, t + TL * m, x, N)
+                 : mk((T)1, (T)0);
+  }
+  // Only the shared-factor class loops over fields: in the others a loop would let
+  // the compiler hoist the per-element factor out of it (all E values live at once).
+  const int bend = (MIDK == 3) ? b1 : b0 + 1;
+  for (int b = b0; b < bend; ++b) {
+    // uniform base (SGPRs) + one 32-bit per-lane offset: every load/store of the
+    // strip addresses as base + lane offset + compile-time constant
+    const long long ubase = (long long)b * a.fstride + x0;
+    cx<T> v[E];
+    const cx<T>* __restrict__ src = a.src + ubase;
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = src[lo + (unsigned)(TL * N * m)];
+    if constexpr ((MODE & MODE_FWD) != 0) line_fft<T, N, E, false>(v, t, acc, a.tw + (N - 2));
+    if constexpr (MIDK == 3) {
+      if (mid) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], h[m]);
+      }
+    } else if constexpr (MIDK == 1) {
+      if (mid) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = spectral_mid<T, false>(a, v[m], b, t + TL * m, x, N);
+      }
+    } else if constexpr (MIDK == 2) {
+      if (mid) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = spectral_mid<T, true>(a, v[m], b, t + TL * m, x, N);
       }
     }
-  }
-  // Keep the inverse transform's twiddle loads below the spectral stage: hoisted above
-  // it they stay live across it and cost tens of registers.
-  asm volatile("" ::: "memory");
-  if constexpr ((MODE & MODE_INV) != 0) line_fft<T, N, E, true>(v, t, acc, a.tw_inv + (N - 2));
-  cx<T>* __restrict__ dst = a.dst + ubase;
-  const bool sc = (a.flags & F_SCALE) != 0;
+    if constexpr (MIDK > 0) {
+      if (spec_out) {
 #pragma unroll
-  for (int m = 0; m < E; ++m) {
-    cx<T> o = v[m];
-    if (sc) o = mk(o.re * a.scale, o.im * a.scale);
-    dst[lo + (unsigned)(TL * N * m)] = o;
+        for (int m = 0; m < E; ++m) a.spec_out[ubase + (lo + (unsigned)(TL * N * m))] = v[m];
+      }
+    }
+    // Keep the inverse transform's twiddle loads below the spectral stage: hoisted
+    // above it they stay live across it and cost registers.
+    asm volatile("" ::: "memory");
+    if constexpr ((MODE & MODE_INV) != 0)
+      line_fft<T, N, E, true>(v, t, acc, a.tw_inv + (N - 2));
+    cx<T>* __restrict__ dst = a.dst + ubase;
+    const T sc = (a.flags & F_SCALE) ? a.scale : (T)1;
+#pragma unroll
+    for (int m = 0; m < E; ++m) dst[lo + (unsigned)(TL * N * m)] = mk(v[m].re * sc, v[m].im * sc);
   }
 }
 
@@ -428,11 +552,10 @@ __device__ __forceinline__ void small_body(const PassArgs<T>& a, int b, int mode
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = src[t + TL * m];
     if (a.flags & F_SPATIAL) {
+      spatial_pro_all<T, E>(a, v, b, y, t, TL, N);
+      if (a.flags & F_FIELD_OUT) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const int x = t + TL * m;
-        v[m] = spatial_pro(a, v[m], b, y, x, N);
-        if (a.flags & F_FIELD_OUT) a.field_out[fb + (long long)y * N + x] = v[m];
+        for (int m = 0; m < E; ++m) a.field_out[fb + (long long)y * N + t + TL * m] = v[m];
       }
     }
     LineAcc<T, PS> acc{lds + y * P};
@@ -474,13 +597,10 @@ __device__ __forceinline__ void small_body(const PassArgs<T>& a, int b, int mode
       line_fft<T, N, E, true>(v, t, acc, twi);
     }
     cx<T>* __restrict__ dst = a.dst + fb + (long long)y * N;
+    spatial_epi_all<T, E>(a, v, b, y, t, TL, N, fb + (long long)y * N + t, true);
+    if ((a.flags & F_NO_STORE) == 0) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const int x = t + TL * m;
-      const cx<T> o = (a.flags & (F_SCALE | F_GRAD | F_INTENSITY))
-                          ? spatial_epi(a, v[m], b, y, x, N, fb + (long long)y * N + x)
-                          : v[m];
-      if ((a.flags & F_NO_STORE) == 0) dst[x] = o;
+      for (int m = 0; m < E; ++m) dst[t + TL * m] = v[m];
     }
   }
 }

@@ -29,6 +29,7 @@ template <typename T>
 struct ColVar {
   ColLaunch<T> fn;
   int E, W, threads, lds;
+  bool hoist;  // transfer function computed once per group of fields
 };
 template <typename T>
 struct SmallVar {
@@ -36,7 +37,7 @@ struct SmallVar {
   int E, threads, lds;
 };
 
-constexpr int kMaxVariants = 6;
+constexpr int kMaxVariants = 8;
 constexpr int kLogMax = 13;  // n <= 8192
 
 template <typename T>
