@@ -37,6 +37,8 @@ def parse():
                    help="time budget of the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--autotune", action="store_true", help="pick kernel variants first")
+    p.add_argument("--no-extra", action="store_true",
+                   help="skip the uneven-batch and speckle-chain legs")
     return p.parse_args()
 
 
@@ -86,6 +88,70 @@ def trace_stats(lib, L):
     return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(cnt.value)}
 
 
+def extra_legs(torch, P, L, lib, dev):
+    """Two more workloads, timed the same way (inputs resident, HIP-event kernel times):
+    the uneven batch of 4096 fields in one persistent launch per sweep, captured in a
+    HIP graph, and the 10-step speckle chain at 1024^2."""
+    out = {}
+    # uneven batch: 2048 x 128^2 + 1536 x 256^2 + 512 x 512^2, complex64, z ~ U[10 um, 1 mm]
+    g = torch.Generator(device="cpu").manual_seed(1)
+    sizes = [128] * 2048 + [256] * 1536 + [512] * 512
+    zs = (torch.rand(len(sizes), generator=g, dtype=torch.float64) * (0.1 - 10e-4) + 10e-4).tolist()
+    pools = {}
+    for nn in (128, 256, 512):
+        cnt = sizes.count(nn)
+        pools[nn] = torch.randn((cnt, nn, nn), dtype=torch.complex64, generator=g).to(dev)
+    idx = {128: 0, 256: 0, 512: 0}
+    srcs = []
+    for nn in sizes:
+        srcs.append(pools[nn][idx[nn]])
+        idx[nn] += 1
+    ub = P.UnevenBatch(srcs, srcs, zs, 200e-4, 532e-7)   # in place
+    graph = ub.capture_graph()
+    for _ in range(2):
+        graph.replay()
+    torch.cuda.synchronize()
+    K = 10
+    L.check(lib.lp2d_trace_enable(1))
+    ub.run()
+    torch.cuda.synchronize()
+    L.check(lib.lp2d_trace_enable(0))
+    sweeps = trace_stats(lib, L)
+    t0 = time.perf_counter()
+    for _ in range(K):
+        graph.replay()
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / K
+    alg = 16.0 * sum(nn * nn for nn in sizes)
+    out["uneven_batch"] = {
+        "workload": "4096 fields (2048x128^2 + 1536x256^2 + 512x512^2) complex64, per-field z, "
+                    "area 200 um, 532 nm; three persistent launches per call, HIP graph replay",
+        "value": round(len(sizes) / el, 1), "unit": "field-propagations/s",
+        "ms_per_call": round(el * 1e3, 4), "alg_bytes": int(alg),
+        "end_to_end_GBs": round(alg / el / 1e9, 1),
+        "end_to_end_frac": round(alg / el / 1e9 / HBM_PEAK_GBS, 4),
+        "sweeps_us": {k: round(ms * 1e3 / c, 1) for k, (c, ms) in sweeps.items()}}
+    del ub, graph, srcs, pools
+    # speckle chain: 1024^2, 10 x [propagate 50 um, NA 0.2 aperture], field to field
+    n, area, wl = 1024, 400e-4, 632e-7
+    u = torch.randn((8, n, n), dtype=torch.complex64, generator=g).to(dev)
+    for _ in range(2):
+        P.propagate(u, 50e-4, area, wl, aperture_radius=0.2 / wl, out=u)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        for _ in range(10):
+            P.propagate(u, 50e-4, area, wl, aperture_radius=0.2 / wl, out=u)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / 5
+    out["speckle_chain"] = {
+        "workload": "8 speckle fields 1024^2 complex64, 10 x [propagate 50 um, NA 0.2 aperture], "
+                    "in place",
+        "value": round(8 * 10 / el, 1), "unit": "field-propagations/s",
+        "ms_per_chain": round(el * 1e3, 4)}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -121,15 +187,21 @@ def main():
         P.propagate_adjoint(gout, u, z, area, wl, phase=phi, out=gin)
 
     def timed(step, K, W):
+        """Wall time of K steps (untraced), then per-kernel times from a traced pass of
+        min(K, 10) more steps: HIP events around every launch perturb back-to-back
+        kernels, so they stay out of the measurement of `value`."""
         for _ in range(W):
             step()
         torch.cuda.synchronize()
-        L.check(lib.lp2d_trace_enable(1))
         t0 = time.perf_counter()
         for _ in range(K):
             step()
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
+        L.check(lib.lp2d_trace_enable(1))
+        for _ in range(min(K, 10)):
+            step()
+        torch.cuda.synchronize()
         L.check(lib.lp2d_trace_enable(0))
         return el, trace_stats(lib, L)
 
@@ -172,6 +244,8 @@ def main():
                 "copy_GBs": round(copy_gbs, 1)}
     kernels = {k: {"launches": c, "avg_us": round(ms * 1e3 / c, 2)} for k, (c, ms) in kt.items()}
 
+    extra = {} if args.no_extra else extra_legs(torch, P, L, lib, dev)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(n, args.cpu_seconds)
@@ -202,6 +276,7 @@ def main():
         "kernels_fwd_adjoint": {k: {"launches": c, "avg_us": round(ms * 1e3 / c, 2)}
                                 for k, (c, ms) in kt2.items()},
         "cpu_baseline": cpu,
+        "workloads": extra,
     }
     print(json.dumps(line))
 

@@ -166,16 +166,16 @@ typedef struct lp2d_field_desc {
 
 typedef struct lp2d_batch lp2d_batch;
 
-/* Builds the size-bucketed work tables of an uneven batch (allocates and uploads
- * them; call outside graph capture).  dtype applies to all fields. */
+/* Groups the fields by size and uploads, per size, device tables of their source and
+ * destination pointers and optics (allocates; call outside graph capture).  dtype
+ * applies to all fields; 2 <= n <= 8192, powers of two. */
 int lp2d_batch_create(const lp2d_field_desc* descs, int count, int dtype, lp2d_batch** out);
-/* Runs every field-propagation of the batch: three persistent launches (the
- * single-tile sizes finish in the first).  Allocates nothing, so it can be
- * captured in a hipGraph. */
+/* Propagates every field src -> dst: per size, the tuned kernels of that size read
+ * the tables (one launch when the whole field fits in LDS, three sweeps otherwise),
+ * largest size first.  Allocates nothing, so it can be captured in a hipGraph.  Must
+ * run on the device the batch was created on. */
 int lp2d_batch_run(lp2d_batch* batch, void* stream);
 int lp2d_batch_destroy(lp2d_batch* batch);
-/* Workgroups per persistent launch (0 = one per CU times occupancy). */
-int lp2d_batch_set_grid(lp2d_batch* batch, int workgroups);
 
 /* ---------------------------------------------------------------- tracing */
 /* Kernel timing on the launch stream: while enabled, every kernel launch of the

@@ -4,6 +4,7 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -688,7 +689,51 @@ int op_adjoint(int n, int lg, int batch, const void* gout, const void* u, const 
   return LP2D_OK;
 }
 
+// ----------------------------------------------------------------- uneven batches
+// Fields are grouped by size; each size bucket keeps device tables of its fields'
+// source / destination pointers and optics, and runs through the tuned per-size
+// kernels (one launch for n <= 128 c64, three sweeps otherwise).
+struct Bucket {
+  int n = 0, lg = 0, count = 0;
+  int flags = 0;
+  void* d_src = nullptr;
+  void* d_dst = nullptr;
+  FieldOpt* d_opt = nullptr;
+};
+
+template <typename T>
+int run_bucket(const Bucket& bk, hipStream_t s) {
+  DeviceTables* t;
+  if (int rc = current_tables(&t)) return rc;
+  PassArgs<T> a = base_args<T>(t, bk.n, bk.count);
+  a.fstride = 0;
+  a.fopt = bk.d_opt;
+  a.src_tab = (const cx<T>* const*)bk.d_src;
+  a.dst_tab = (cx<T>* const*)bk.d_dst;
+  const T sc = (T)(1.0 / ((double)bk.n * bk.n));
+  if (use_small<T>(bk.lg)) {
+    a.flags = bk.flags | F_SCALE;
+    a.scale = sc;
+    return run_small<T>(a, bk.lg, MODE_FWD | MODE_INV, s);
+  }
+  a.flags = 0;
+  if (int rc = run_rows<T>(a, bk.lg, false, s)) return rc;
+  a.src_tab = (const cx<T>* const*)bk.d_dst;  // later sweeps work in the destinations
+  a.flags = bk.flags;
+  if (int rc = run_cols<T>(a, bk.lg, MODE_FWD | MODE_INV, s)) return rc;
+  a.flags = F_SCALE;
+  a.scale = sc;
+  return run_rows<T>(a, bk.lg, true, s);
+}
+
 }  // namespace
+
+struct lp2d_batch {
+  int device = 0;
+  int dtype = 0;
+  void* d_mem = nullptr;
+  std::vector<Bucket> buckets;
+};
 
 // ===================================================================== C ABI
 extern "C" {
@@ -830,6 +875,97 @@ int lp2d_propagate_adjoint(int n, int batch, int dtype, const void* grad_out, co
              : op_adjoint<double>(n, lg, batch, grad_out, field_in, phase, phase_stride_b, optics, grad_in, grad_phase, adj_flags, s);
 }
 
+int lp2d_batch_create(const lp2d_field_desc* descs, int count, int dtype, lp2d_batch** out) {
+  if (!out) return fail(LP2D_E_ARG, "null output handle");
+  *out = nullptr;
+  if (!descs || count < 1) return fail(LP2D_E_ARG, "need at least one field descriptor");
+  if (dtype != LP2D_C64 && dtype != LP2D_C128) return fail(LP2D_E_DTYPE, "unknown dtype %d", dtype);
+  std::vector<std::vector<int>> by_lg(kLogMax + 1);
+  for (int i = 0; i < count; ++i) {
+    const lp2d_field_desc& d = descs[i];
+    const int lg = log2_exact(d.n);
+    if (lg < 1 || lg > kLogMax)
+      return fail(LP2D_E_SIZE, "field %d: n=%d must be a power of two in [2, %d]", i, d.n, kMaxN);
+    if (!d.src || !d.dst) return fail(LP2D_E_ARG, "field %d: null pointer", i);
+    if (!(d.area_size > 0) || !(d.wl > 0)) return fail(LP2D_E_ARG, "field %d: area_size and wl must be > 0", i);
+    by_lg[lg].push_back(i);
+  }
+  DeviceTables* t;
+  if (int rc = current_tables(&t)) return rc;
+  lp2d_batch* b = new lp2d_batch();
+  hipGetDevice(&b->device);
+  b->dtype = dtype;
+  // one allocation: per bucket [src ptrs | dst ptrs | optics]
+  size_t bytes = 0;
+  for (int lg = kLogMax; lg >= 1; --lg)
+    if (!by_lg[lg].empty()) bytes += by_lg[lg].size() * (2 * sizeof(void*) + sizeof(FieldOpt));
+  std::vector<unsigned char> host(bytes);
+  size_t off = 0;
+  for (int lg = kLogMax; lg >= 1; --lg) {  // largest fields first
+    const std::vector<int>& ids = by_lg[lg];
+    if (ids.empty()) continue;
+    Bucket bk;
+    bk.n = 1 << lg;
+    bk.lg = lg;
+    bk.count = (int)ids.size();
+    bool any_ap = false;
+    void** sp = (void**)(host.data() + off);
+    void** dp = sp + ids.size();
+    FieldOpt* op = (FieldOpt*)(dp + ids.size());
+    for (size_t k = 0; k < ids.size(); ++k) {
+      const lp2d_field_desc& d = descs[ids[k]];
+      const double dL = d.area_size / d.n, k0 = 1.0 / d.wl;
+      sp[k] = const_cast<void*>(d.src);
+      dp[k] = d.dst;
+      op[k].k0sq = k0 * k0;
+      op[k].inv_ndL = 1.0 / ((double)d.n * dL);
+      op[k].z = d.z;
+      op[k].ap_r2 = d.aperture_radius > 0.0 ? d.aperture_radius * d.aperture_radius : HUGE_VAL;
+      any_ap = any_ap || d.aperture_radius > 0.0;
+    }
+    bk.flags = F_H | (any_ap ? F_APERTURE : 0);
+    bk.d_src = (void*)(off);
+    bk.d_dst = (void*)(off + ids.size() * sizeof(void*));
+    bk.d_opt = (FieldOpt*)(off + 2 * ids.size() * sizeof(void*));
+    off += ids.size() * (2 * sizeof(void*) + sizeof(FieldOpt));
+    b->buckets.push_back(bk);
+  }
+  hipError_t e = hipMalloc(&b->d_mem, bytes);
+  if (e == hipSuccess) e = hipMemcpy(b->d_mem, host.data(), bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hipFree(b->d_mem);
+    delete b;
+    return hip_fail(e, "batch table upload");
+  }
+  for (Bucket& bk : b->buckets) {  // offsets -> device addresses
+    bk.d_src = (char*)b->d_mem + (size_t)bk.d_src;
+    bk.d_dst = (char*)b->d_mem + (size_t)bk.d_dst;
+    bk.d_opt = (FieldOpt*)((char*)b->d_mem + (size_t)bk.d_opt);
+  }
+  *out = b;
+  return LP2D_OK;
+}
+
+int lp2d_batch_run(lp2d_batch* b, void* stream) {
+  if (!b) return fail(LP2D_E_ARG, "null batch");
+  int dev = 0;
+  hipGetDevice(&dev);
+  if (dev != b->device) return fail(LP2D_E_ARG, "batch was created on device %d, current is %d", b->device, dev);
+  hipStream_t s = (hipStream_t)stream;
+  for (const Bucket& bk : b->buckets) {
+    const int rc = b->dtype == LP2D_C64 ? run_bucket<float>(bk, s) : run_bucket<double>(bk, s);
+    if (rc) return rc;
+  }
+  return LP2D_OK;
+}
+
+int lp2d_batch_destroy(lp2d_batch* b) {
+  if (!b) return LP2D_OK;
+  hipFree(b->d_mem);
+  delete b;
+  return LP2D_OK;
+}
+
 int lp2d_sum_planes(int64_t count, int batch, const float* in, float* out, void* stream) {
   if (count < 0 || batch < 1) return fail(LP2D_E_SIZE, "bad count or batch");
   if (!in || !out) return fail(LP2D_E_ARG, "null pointer");
@@ -938,21 +1074,6 @@ int lp2d_trace_collect(lp2d_trace_stat* stats, int max_stats, int* n_stats) {
   *n_stats = (int)agg.size();
   for (int i = 0; i < (int)agg.size() && i < max_stats; ++i) stats[i] = agg[i];
   return rc;
-}
-
-// Internal (used by lp2d_batch.hip): bracket a launch with trace events.
-void* lp2d_trace_begin(void* stream, const char* name) {
-  if (!g_trace_on) return nullptr;
-  return new TraceScope((hipStream_t)stream, name);
-}
-void lp2d_trace_end(void* token) { delete (TraceScope*)token; }
-
-// Internal (used by lp2d_batch.hip): the c64 twiddle tables of the current device.
-int lp2d_twiddles_f32(const void** out) {
-  DeviceTables* t;
-  if (int rc = current_tables(&t)) return rc;
-  *out = t->twf;
-  return LP2D_OK;
 }
 
 }  // extern "C"

@@ -13,8 +13,12 @@ constexpr int floor_pow2(int n) { return n < 2 ? 1 : 2 * floor_pow2(n >> 1); }
 constexpr int kLdsBudget = 160 * 1024 - 2048;
 
 // ----------------------------------------------------------------- kernels
+// Every pass kernel asks for at least two waves per SIMD: the compiler then budgets at
+// most 256 registers per lane (one more and a 256-thread workgroup drops to a single
+// wave per SIMD, halving the latency hiding).
+#define LP2D_OCC __attribute__((amdgpu_waves_per_eu(2)))
 template <typename T, int N, int E, int LINES, bool INV, int PS, bool FEAT>
-__global__ __launch_bounds__((N / E) * LINES) void k_rows(PassArgs<T> a) {
+__global__ __launch_bounds__((N / E) * LINES) LP2D_OCC void k_rows(PassArgs<T> a) {
   __shared__ cx<T> lds[LINES * PaddedLen<N, PS>::value];
   constexpr int TPF = (N + LINES - 1) / LINES;  // row tiles per field
   const int tile = blockIdx.x;
@@ -24,7 +28,7 @@ __global__ __launch_bounds__((N / E) * LINES) void k_rows(PassArgs<T> a) {
 }
 
 template <typename T, int N, int E, int LINES, int PS>
-__global__ __launch_bounds__((N / E) * LINES) void k_rows_grad(PassArgs<T> a, int groups,
+__global__ __launch_bounds__((N / E) * LINES) LP2D_OCC void k_rows_grad(PassArgs<T> a, int groups,
                                                                 float* partial) {
   __shared__ cx<T> lds[LINES * PaddedLen<N, PS>::value];
   constexpr int TPF = (N + LINES - 1) / LINES;
@@ -38,7 +42,7 @@ __global__ __launch_bounds__((N / E) * LINES) void k_rows_grad(PassArgs<T> a, in
 
 // Grid: (N/W strips) x (field groups of `per` fields).
 template <typename T, int N, int E, int W, int MODE, int PS, int MIDK>
-__global__ __launch_bounds__((N / E) * W) void k_cols(PassArgs<T> a, int per) {
+__global__ __launch_bounds__((N / E) * W) LP2D_OCC void k_cols(PassArgs<T> a, int per) {
   __shared__ cx<T> lds[W * PaddedLen<N, PS>::value];
   constexpr int SPF = N / W;  // column strips per field
   const int g = gridDim.x, i = blockIdx.x;
@@ -125,8 +129,9 @@ template <typename T, int N, int E, int W, int PS, bool HOIST>
 hipError_t launch_cols(const PassArgs<T>& a, int mode, hipStream_t s) {
   constexpr int threads = (N / E) * W;
   constexpr int SPF = N / W;
-  constexpr int MH = HOIST ? 3 : 1;  // spectral-stage class of the H / aperture path
-  const bool shared_h = HOIST && (a.flags & F_MID) && !(a.flags & (F_SMASK0 | F_SMASK1));
+  // the shared-factor class needs one set of optics for the whole call
+  const bool hoist = HOIST && a.fopt == nullptr;
+  const bool shared_h = hoist && (a.flags & F_MID) && !(a.flags & (F_SMASK0 | F_SMASK1));
   int per = shared_h ? (int)((long long)a.batch * SPF / kColBlocks) : 1;
   if (per < 1) per = 1;
   if (per > 8) per = 8;
@@ -143,11 +148,23 @@ hipError_t launch_cols(const PassArgs<T>& a, int mode, hipStream_t s) {
       k_cols<T, N, E, W, MODE_INV, PS, 0><<<grid, threads, 0, s>>>(a, per);
       break;
     case MODE_INV * 4 + 1:
-      k_cols<T, N, E, W, MODE_INV, PS, MH><<<grid, threads, 0, s>>>(a, per);
+      if constexpr (HOIST) {
+        if (hoist) {
+          k_cols<T, N, E, W, MODE_INV, PS, 3><<<grid, threads, 0, s>>>(a, per);
+          break;
+        }
+      }
+      k_cols<T, N, E, W, MODE_INV, PS, 1><<<grid, threads, 0, s>>>(a, per);
       break;
     case FI * 4 + 0:
     case FI * 4 + 1:
-      k_cols<T, N, E, W, FI, PS, MH><<<grid, threads, 0, s>>>(a, per);
+      if constexpr (HOIST) {
+        if (hoist) {
+          k_cols<T, N, E, W, FI, PS, 3><<<grid, threads, 0, s>>>(a, per);
+          break;
+        }
+      }
+      k_cols<T, N, E, W, FI, PS, 1><<<grid, threads, 0, s>>>(a, per);
       break;
     case FI * 4 + 2:
       k_cols<T, N, E, W, FI, PS, 2><<<grid, threads, 0, s>>>(a, per);

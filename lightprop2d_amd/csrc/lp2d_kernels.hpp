@@ -48,6 +48,11 @@ struct MaskD {
   long long sb, sy, sx;
 };
 
+// Per-field optics of an uneven batch (device table, one entry per field).
+struct FieldOpt {
+  double k0sq, inv_ndL, z, ap_r2;
+};
+
 template <typename T>
 struct PassArgs {
   const cx<T>* tw;      // concatenated twiddle tables, exp(-2*pi*i*k/N)
@@ -79,7 +84,34 @@ struct PassArgs {
   const cx<T>* u_saved;
   long long u_sb;
   float* grad_phase;
+  // uneven batches: per-field base pointers and optics (null = strided batch)
+  const cx<T>* const* src_tab;
+  cx<T>* const* dst_tab;
+  const FieldOpt* fopt;
 };
+
+// Base of field b's source / destination plane.
+template <typename T>
+__device__ __forceinline__ const cx<T>* field_src(const PassArgs<T>& a, int b) {
+  return a.src_tab ? a.src_tab[b] : a.src + (long long)b * a.fstride;
+}
+template <typename T>
+__device__ __forceinline__ cx<T>* field_dst(const PassArgs<T>& a, int b) {
+  return a.dst_tab ? a.dst_tab[b] : a.dst + (long long)b * a.fstride;
+}
+// The call's optics, or field b's when the batch carries a per-field table.
+template <typename T>
+__device__ __forceinline__ PassArgs<T> field_optics(const PassArgs<T>& a, int b) {
+  PassArgs<T> r = a;
+  if (a.fopt) {
+    const FieldOpt o = a.fopt[b];
+    r.k0sq = o.k0sq;
+    r.inv_ndL = o.inv_ndL;
+    r.z = o.z;
+    r.ap_r2 = o.ap_r2;
+  }
+  return r;
+}
 
 // ----------------------------------------------------------------- twiddles
 // Concatenated per-size tables (PassArgs::tw): the table of size N (a power of two)
@@ -374,10 +406,10 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
   const bool valid = l < nrows;
   const long long frow = (long long)b * a.fstride + (long long)y * N;
   // uniform base (SGPRs) + one 32-bit per-lane offset
-  const long long ubase = (long long)b * a.fstride + (long long)y0 * N;
+  const long long ubase = (long long)y0 * N;
   const unsigned lo = (unsigned)((y - y0) * N + t);
   cx<T> v[E];
-  const cx<T>* __restrict__ src = a.src + ubase;
+  const cx<T>* __restrict__ src = field_src(a, b) + ubase;
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = src[lo + (unsigned)(TL * m)];
   if constexpr (FEAT) {
@@ -392,7 +424,7 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
   }
   LineAcc<T, PS> acc{lds + l * L};
   line_fft<T, N, E, INV>(v, t, acc, (INV ? a.tw_inv : a.tw) + (N - 2));
-  cx<T>* __restrict__ dst = a.dst + ubase;
+  cx<T>* __restrict__ dst = field_dst(a, b) + ubase;
   if constexpr (FEAT) {
     spatial_epi_all<T, E>(a, v, b, y, t, TL, N, frow + t, valid);
     if ((a.flags & F_NO_STORE) == 0) {
@@ -475,7 +507,7 @@ __device__ __forceinline__ void cols_body(const PassArgs<T>& a, int b0, int b1, 
   const bool mid = (a.flags & F_MID) != 0;
   const bool spec_out = (a.flags & F_SPEC_OUT) != 0;
   cx<T> h[MIDK == 3 ? E : 1];
-  if constexpr (MIDK == 3) {
+  if constexpr (MIDK == 3) {  // (never used with per-field optics)
 #pragma unroll
     for (int m = 0; m < E; ++m)
       h[m] = mid ? spectral_mid<T, false>(a, mk((T)1, (T)0), 0, t + TL * m, x, N)
@@ -489,7 +521,9 @@ __device__ __forceinline__ void cols_body(const PassArgs<T>& a, int b0, int b1, 
     // strip addresses as base + lane offset + compile-time constant
     const long long ubase = (long long)b * a.fstride + x0;
     cx<T> v[E];
-    const cx<T>* __restrict__ src = a.src + ubase;
+    // the shared-factor class never runs on per-field tables (their optics differ)
+    const cx<T>* __restrict__ src = (MIDK == 3 ? a.src + (long long)b * a.fstride
+                                               : field_src(a, b)) + x0;
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = src[lo + (unsigned)(TL * N * m)];
     if constexpr ((MODE & MODE_FWD) != 0) line_fft<T, N, E, false>(v, t, acc, a.tw + (N - 2));
@@ -500,8 +534,9 @@ __device__ __forceinline__ void cols_body(const PassArgs<T>& a, int b0, int b1, 
       }
     } else if constexpr (MIDK == 1) {
       if (mid) {
+        const PassArgs<T> ab = field_optics(a, b);
 #pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = spectral_mid<T, false>(a, v[m], b, t + TL * m, x, N);
+        for (int m = 0; m < E; ++m) v[m] = spectral_mid<T, false>(ab, v[m], b, t + TL * m, x, N);
       }
     } else if constexpr (MIDK == 2) {
       if (mid) {
@@ -520,7 +555,7 @@ __device__ __forceinline__ void cols_body(const PassArgs<T>& a, int b0, int b1, 
     asm volatile("" ::: "memory");
     if constexpr ((MODE & MODE_INV) != 0)
       line_fft<T, N, E, true>(v, t, acc, a.tw_inv + (N - 2));
-    cx<T>* __restrict__ dst = a.dst + ubase;
+    cx<T>* __restrict__ dst = (MIDK == 3 ? a.dst + (long long)b * a.fstride : field_dst(a, b)) + x0;
     const T sc = (a.flags & F_SCALE) ? a.scale : (T)1;
 #pragma unroll
     for (int m = 0; m < E; ++m) dst[lo + (unsigned)(TL * N * m)] = mk(v[m].re * sc, v[m].im * sc);
@@ -544,11 +579,13 @@ __device__ __forceinline__ void small_body(const PassArgs<T>& a, int b, int mode
   const cx<T>* tw = a.tw + (N - 2);
   const cx<T>* twi = a.tw_inv + (N - 2);
   const long long fb = (long long)b * a.fstride;
+  const cx<T>* fsrc = field_src(a, b);
+  cx<T>* fdst = field_dst(a, b);
   cx<T> v[E];
   // phase 1: rows (load, spatial prologue, forward row FFT)
   {
     const int y = tid / TL, t = tid % TL;
-    const cx<T>* __restrict__ src = a.src + fb + (long long)y * N;
+    const cx<T>* __restrict__ src = fsrc + (long long)y * N;
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = src[t + TL * m];
     if (a.flags & F_SPATIAL) {
@@ -573,10 +610,11 @@ __device__ __forceinline__ void small_body(const PassArgs<T>& a, int b, int mode
     __syncthreads();  // the column FFT below overwrites rows other threads still read
     if (mode & MODE_FWD) line_fft<T, N, E, false>(v, t, acc, tw);
     if (a.flags & F_MID) {
+      const PassArgs<T> ab = field_optics(a, b);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int ky = t + TL * m;
-        v[m] = spectral_mid(a, v[m], b, ky, x, N);
+        v[m] = spectral_mid(ab, v[m], b, ky, x, N);
         if (a.flags & F_SPEC_OUT) a.spec_out[fb + (long long)ky * N + x] = v[m];
       }
     }
@@ -596,7 +634,7 @@ __device__ __forceinline__ void small_body(const PassArgs<T>& a, int b, int mode
       __syncthreads();
       line_fft<T, N, E, true>(v, t, acc, twi);
     }
-    cx<T>* __restrict__ dst = a.dst + fb + (long long)y * N;
+    cx<T>* __restrict__ dst = fdst + (long long)y * N;
     spatial_epi_all<T, E>(a, v, b, y, t, TL, N, fb + (long long)y * N + t, true);
     if ((a.flags & F_NO_STORE) == 0) {
 #pragma unroll

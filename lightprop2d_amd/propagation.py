@@ -281,9 +281,9 @@ def intensity(field: torch.Tensor, out: Optional[torch.Tensor] = None):
 
 
 class UnevenBatch:
-    """Many independent complex64 fields of different sizes (64..4096), distances and
-    optics, propagated by three persistent launches per call (whole-field tiles for
-    n <= 128, row/column tiles otherwise).
+    """Many independent fields of different sizes (powers of two up to 8192), distances
+    and optics, propagated per size by the tuned kernels reading per-field pointer and
+    optics tables (one launch for sizes that fit in LDS, three sweeps otherwise).
 
     Fields are propagated src -> dst (dst may be src).  `run()` allocates nothing and
     can be captured in a CUDA/HIP graph (`capture_graph()`).
@@ -304,8 +304,12 @@ class UnevenBatch:
         descs = (L.lp2d_field_desc * m)()
         for i, (s, d) in enumerate(zip(srcs, dsts)):
             n, b, dt = _field_geom(s, f"srcs[{i}]")
-            if b != 1 or s.dim() != 2 or dt != L.LP2D_C64:
-                raise Lp2dError("uneven batches take complex64 (n, n) fields")
+            if b != 1 or s.dim() != 2:
+                raise Lp2dError("uneven batches take (n, n) fields")
+            if i == 0:
+                dtype0 = dt
+            elif dt != dtype0:
+                raise Lp2dError("all fields of an uneven batch share one dtype")
             if d.shape != s.shape or d.dtype != s.dtype or not d.is_contiguous():
                 raise Lp2dError(f"dsts[{i}] must match srcs[{i}]")
             descs[i] = L.lp2d_field_desc(n, 0, s.data_ptr(), d.data_ptr(), float(areas[i]),
@@ -314,13 +318,10 @@ class UnevenBatch:
         self.device = srcs[0].device
         self._h = C.c_void_p()
         with torch.cuda.device(self.device):
-            L.check(lib.lp2d_batch_create(descs, m, L.LP2D_C64, C.byref(self._h)), "batch_create")
+            L.check(lib.lp2d_batch_create(descs, m, dtype0, C.byref(self._h)), "batch_create")
         self.count = m
         self.elements = sum(s.numel() for s in srcs)
         self._graph = None
-
-    def set_grid(self, workgroups: int):
-        L.check(L.lib().lp2d_batch_set_grid(self._h, int(workgroups)), "batch_set_grid")
 
     def run(self):
         with torch.cuda.device(self.device):

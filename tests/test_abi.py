@@ -75,7 +75,9 @@ def test_argument_errors_before_any_device_work(lib):
     d = (_lib.lp2d_field_desc * 1)(_lib.lp2d_field_desc(48, 0, 16, 16, 1.0, 1e-4, 0.1, 0.0))
     h = C.c_void_p()
     assert lib.lp2d_batch_create(d, 1, 0, C.byref(h)) == _lib.LP2D_E_SIZE
-    assert lib.lp2d_batch_create(d, 1, 1, C.byref(h)) == _lib.LP2D_E_DTYPE
+    assert lib.lp2d_batch_create(d, 1, 7, C.byref(h)) == _lib.LP2D_E_DTYPE
+    d[0] = _lib.lp2d_field_desc(64, 0, None, 16, 1.0, 1e-4, 0.1, 0.0)
+    assert lib.lp2d_batch_create(d, 1, 0, C.byref(h)) == _lib.LP2D_E_ARG
 
 
 def test_python_layer_rejects_cpu_tensors():

@@ -419,3 +419,27 @@ def test_reference_gaussian_diffraction_script_on_gpu():
     ph = np.array(phases) - 2 * np.pi * (np.trunc(beam.k0 * z_grid) - beam.k0 * z_grid)
     ph = (ph + np.pi) % (2 * np.pi) - np.pi
     np.testing.assert_allclose(ph, np.arctan(wl0 * z_grid / np.pi / R ** 2), atol=2e-3)
+
+
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128], ids=["c64", "c128"])
+def test_uneven_batch_per_field_optics(dt):
+    """Per-field z, area, wavelength and aperture (some fields without one), sizes
+    from 16 to 1024 in one batch, against the oracle field by field."""
+    from lightprop2d_amd import propagation as P
+    rs = np.random.RandomState(3)
+    sizes = [1024, 256, 16, 512, 64, 128, 256, 32]
+    srcs, dsts, zs, areas, wls, aps, host_in = [], [], [], [], [], [], []
+    for i, n in enumerate(sizes):
+        f = cplx(rs, (n, n), dt)
+        host_in.append(f)
+        srcs.append(to_dev(f))
+        dsts.append(torch.empty_like(srcs[-1]))
+        zs.append(rs.uniform(1e-4, 0.05))
+        areas.append(rs.uniform(100e-4, 400e-4))
+        wls.append(rs.uniform(400e-7, 1100e-7))
+        aps.append(0.0 if i % 3 == 0 else 0.3 / (areas[-1] / n))
+    P.UnevenBatch(srcs, dsts, zs, areas, wls, aps).run()
+    torch.cuda.synchronize()
+    for f, d, z, a, w, ap in zip(host_in, dsts, zs, areas, wls, aps):
+        exp = oracle.propagate_field(f, a, w, z, aperture_radius=ap if ap else None)
+        assert rel_l2(host(d), exp) < tol(dt)
