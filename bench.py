@@ -90,8 +90,10 @@ def trace_stats(lib, L):
 
 def extra_legs(torch, P, L, lib, dev):
     """Two more workloads, timed the same way (inputs resident, HIP-event kernel times):
-    the uneven batch of 4096 fields in one persistent launch per sweep, captured in a
-    HIP graph, and the 10-step speckle chain at 1024^2."""
+    the uneven batch of 4096 fields (one launch per size bucket and sweep over
+    per-field descriptor tables, captured in a HIP graph), and the 10-step speckle
+    chain at 1024^2.  alg_bytes counts one read and one write of every field (16 B
+    per complex64 element), the least any propagation can move."""
     out = {}
     # uneven batch: 2048 x 128^2 + 1536 x 256^2 + 512 x 512^2, complex64, z ~ U[10 um, 1 mm]
     g = torch.Generator(device="cpu").manual_seed(1)
@@ -125,7 +127,8 @@ def extra_legs(torch, P, L, lib, dev):
     alg = 16.0 * sum(nn * nn for nn in sizes)
     out["uneven_batch"] = {
         "workload": "4096 fields (2048x128^2 + 1536x256^2 + 512x512^2) complex64, per-field z, "
-                    "area 200 um, 532 nm; three persistent launches per call, HIP graph replay",
+                    "area 200 um, 532 nm; size-bucketed launches over descriptor tables, "
+                    "HIP graph replay",
         "value": round(len(sizes) / el, 1), "unit": "field-propagations/s",
         "ms_per_call": round(el * 1e3, 4), "alg_bytes": int(alg),
         "end_to_end_GBs": round(alg / el / 1e9, 1),

@@ -15,6 +15,23 @@
 
 using namespace lp2d;
 
+namespace lp2d {
+int resident_groups(const void* kernel, int threads) {
+  int dev = 0, cus = 0, nb = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus < 1) {
+    (void)hipGetLastError();
+    cus = 256;
+  }
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, threads, 0) != hipSuccess || nb < 1) {
+    (void)hipGetLastError();
+    nb = 1;
+  }
+  return cus * nb;
+}
+}  // namespace lp2d
+
 namespace {
 
 thread_local std::string g_err;
@@ -1031,9 +1048,9 @@ int lp2d_variant_name(int n, int dtype, int kind, int variant, char* buf, int bu
   const int W = dtype ? g_rd->cols[lg][variant].W : g_rf->cols[lg][variant].W;
   const int th = dtype ? g_rd->cols[lg][variant].threads : g_rf->cols[lg][variant].threads;
   const int lds = dtype ? g_rd->cols[lg][variant].lds : g_rf->cols[lg][variant].lds;
-  const bool hoist = dtype ? g_rd->cols[lg][variant].hoist : g_rf->cols[lg][variant].hoist;
-  return snprintf(buf, buflen, "cols E=%d W=%d threads=%d lds=%d%s", E, W, th, lds,
-                  hoist ? " shared-H" : "");
+  const int ck = dtype ? g_rd->cols[lg][variant].kind : g_rf->cols[lg][variant].kind;
+  return snprintf(buf, buflen, "cols E=%d W=%d threads=%d lds=%d%s%s", E, W, th, lds,
+                  (ck & 2) ? " pipelined" : "", (ck & 1) ? " shared-H" : "");
 }
 
 }  // extern "C"

@@ -251,4 +251,84 @@ __device__ __forceinline__ void line_fft(cx<T> (&v)[E], int t, const Acc& acc,
   stockham<T, N, E, INV, 1>(v, t, acc, tw);
 }
 
+// ---------------------------------------------------------------------------
+// Twiddles staged in LDS as one 2-D table per pass: the pass with stride NS and radix
+// R keeps w_{NS*R}^{k*r} at row k, column r, rows padded to R+2 entries (the rows of
+// the lanes of one wave then start in different banks, and stay 16-byte aligned).  A
+// butterfly reads its R-1 twiddles at one per-lane base plus compile-time offsets, and
+// the inverse transform uses the same table conjugated.
+constexpr int twl_radix(int N, int E, int ns) { return N / ns >= E ? E : N / ns; }
+constexpr int twl_off(int N, int E, int NS) {
+  int off = 0;
+  for (int ns = 1; ns < NS; ns *= twl_radix(N, E, ns))
+    if (ns > 1) off += ns * (twl_radix(N, E, ns) + 2);
+  return off;
+}
+template <int N, int E>
+struct TwlLen {
+  static constexpr int value = twl_off(N, E, N) > 0 ? twl_off(N, E, N) : 1;
+};
+
+// Fill the per-pass tables from the concatenated global table (size L at offset L-2).
+template <typename T, int N, int E, int THREADS, int NS = 1>
+__device__ __forceinline__ void twl_fill(cx<T>* twl, const cx<T>* __restrict__ tw, int tid) {
+  if constexpr (NS < N) {
+    constexpr int R = twl_radix(N, E, NS);
+    if constexpr (NS > 1) {
+      constexpr int L = NS * R, P = R + 2, OFF = twl_off(N, E, NS);
+      for (int i = tid; i < L; i += THREADS) {
+        const int k = i / R, r = i % R;
+        twl[OFF + k * P + r] = tw[(L - 2) + k * r];
+      }
+    }
+    twl_fill<T, N, E, THREADS, NS * R>(twl, tw, tid);
+  }
+}
+
+// The Stockham passes above with the twiddles read from the LDS tables.
+template <typename T, int N, int E, bool INV, int NS, typename Acc>
+__device__ __forceinline__ void stockham_l(cx<T> (&v)[E], int t, const Acc& acc,
+                                           const cx<T>* twl) {
+  constexpr int REM = N / NS;
+  constexpr int R = REM >= E ? E : REM;
+  constexpr int B = E / R;
+  constexpr int TL = N / E;
+  constexpr bool LAST = (NS * R == N);
+#pragma unroll
+  for (int q = 0; q < B; ++q) {
+    const int j = t + TL * q;
+    const int k = j & (NS - 1);
+    cx<T> a[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) a[r] = v[q + r * B];
+    if constexpr (NS > 1) {
+      const cx<T>* tp = twl + (twl_off(N, E, NS) + k * (R + 2));
+#pragma unroll
+      for (int r = 1; r < R; ++r) a[r] = INV ? cmulc(a[r], tp[r]) : cmul(a[r], tp[r]);
+    }
+    dft<T, R, INV>(a);
+    if constexpr (LAST) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q + r * B] = a[r];
+    } else {
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc.st(base + r * NS, a[r]);
+    }
+  }
+  if constexpr (!LAST) {
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = acc.ld(t + TL * m);
+    __syncthreads();
+    stockham_l<T, N, E, INV, NS * R>(v, t, acc, twl);
+  }
+}
+
+template <typename T, int N, int E, bool INV, typename Acc>
+__device__ __forceinline__ void line_fft_l(cx<T> (&v)[E], int t, const Acc& acc,
+                                           const cx<T>* twl) {
+  stockham_l<T, N, E, INV, 1>(v, t, acc, twl);
+}
+
 }  // namespace lp2d

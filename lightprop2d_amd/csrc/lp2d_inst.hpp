@@ -56,6 +56,29 @@ __global__ __launch_bounds__((N / E) * W) LP2D_OCC void k_cols(PassArgs<T> a, in
   cols_body<T, N, E, W, MODE, PS, MIDK>(a, b0, b1, x0, lds, threadIdx.x);
 }
 
+// Pipelined column pass: grid (N/W strips) x (field groups of `per` <= kPipeMaxPer
+// fields); the twiddle tables of the strip's passes and the group's field pointers
+// and optics are staged in LDS once per workgroup.
+template <typename T, int N, int E, int W, int PS>
+__global__ __launch_bounds__((N / E) * W) LP2D_OCC void k_cols_pipe(PassArgs<T> a, int per) {
+  __shared__ cx<T> lds[W * PaddedLen<N, PS>::value];
+  __shared__ cx<T> twl[TwlLen<N, E>::value];
+  __shared__ PipeField pf[kPipeMaxPer];
+  constexpr int THREADS = (N / E) * W;
+  static_assert(THREADS >= kPipeMaxPer, "one thread per staged field");
+  constexpr int SPF = N / W;
+  const int g = gridDim.x, i = blockIdx.x;
+  const int blk = (g % 8 == 0) ? (i % 8) * (g / 8) + i / 8 : i;
+  const int grp = blk / SPF;
+  const int x0 = (blk - grp * SPF) * W;
+  const int b0 = grp * per;
+  const int b1 = cmin(a.batch, b0 + per);
+  twl_fill<T, N, E, THREADS>(twl, a.tw, threadIdx.x);
+  pipe_stage_fields(a, b0, b1, pf, threadIdx.x);
+  __syncthreads();
+  cols_pipe_body<T, N, E, W, PS>(a, b0, b1, x0, lds, twl, pf, threadIdx.x);
+}
+
 template <typename T, int N, int E, int PS>
 __global__ __launch_bounds__(N * (N / E)) void k_small(PassArgs<T> a, int mode) {
   __shared__ cx<T> lds[N * SmallPitch<N, PS>::value];
@@ -179,6 +202,35 @@ hipError_t launch_cols(const PassArgs<T>& a, int mode, hipStream_t s) {
   return hipGetLastError();
 }
 
+// The pipelined class covers the fused forward+inverse column pass of the complex64
+// path with transfer function / pupil / spectrum output; anything else (mask
+// multipliers, other evanescent modes, single-direction passes) runs the plain class
+// of the same shape.  Each workgroup takes `per` fields so that the grid is about one
+// round of resident workgroups.
+template <typename T, int N, int E, int W, int PS>
+hipError_t launch_cols_pipe(const PassArgs<T>& a, int mode, hipStream_t s) {
+  constexpr int FI = MODE_FWD | MODE_INV;
+  const bool ok = sizeof(T) == 4 && mode == FI && a.evan == EV_PASS &&
+                  !(a.flags & (F_SMASK0 | F_SMASK1));
+  if constexpr (sizeof(T) == 4) {
+    if (ok) {
+      constexpr int threads = (N / E) * W;
+      constexpr int SPF = N / W;
+      static int slots = 0;  // same value on every thread that races to set it
+      if (slots == 0) slots = resident_groups((const void*)&k_cols_pipe<T, N, E, W, PS>, threads);
+      int groups = slots / SPF;
+      if (groups < 1) groups = 1;
+      if (groups > a.batch) groups = a.batch;
+      int per = (a.batch + groups - 1) / groups;
+      if (per > kPipeMaxPer) per = kPipeMaxPer;
+      groups = (a.batch + per - 1) / per;
+      k_cols_pipe<T, N, E, W, PS><<<(unsigned)(groups * SPF), threads, 0, s>>>(a, per);
+      return hipGetLastError();
+    }
+  }
+  return launch_cols<T, N, E, W, PS, false>(a, mode, s);
+}
+
 template <typename T, int N, int E, int PS>
 hipError_t launch_small(const PassArgs<T>& a, int mode, hipStream_t s) {
   k_small<T, N, E, PS><<<(unsigned)a.batch, N * (N / E), 0, s>>>(a, mode);
@@ -204,7 +256,20 @@ void add_col(Registry<T>& r) {
   int& k = r.ncols[lg];
   if (k >= kMaxVariants) return;
   r.cols[lg][k++] = ColVar<T>{&launch_cols<T, N, E, S::W, S::PS, HOIST>, E, S::W, S::THREADS,
-                              S::LDS, HOIST};
+                              S::LDS, HOIST ? 1 : 0};
+}
+
+template <typename T, int N, int E, int WT>
+void add_col_pipe(Registry<T>& r) {
+  using S = ColShape<T, N, E, WT>;
+  constexpr int lg = ilog2c(N);
+  int& k = r.ncols[lg];
+  if (k >= kMaxVariants) return;
+  constexpr int lds = S::LDS + TwlLen<N, E>::value * (int)sizeof(cx<T>) +
+                      kPipeMaxPer * (int)sizeof(PipeField);
+  if constexpr (lds <= kLdsBudget)
+    r.cols[lg][k++] = ColVar<T>{&launch_cols_pipe<T, N, E, S::W, S::PS>, E, S::W, S::THREADS,
+                                lds, 2};
 }
 
 template <typename T, int N>
@@ -231,6 +296,9 @@ void register_n(Registry<T>& r) {
       add_col<T, N, 32, 8>(r);
       add_col<T, N, 32, 8, true>(r);
       add_col<T, N, 16, 8, true>(r);
+      add_col_pipe<T, N, 32, 8>(r);
+      add_col_pipe<T, N, 16, 8>(r);
+      add_col_pipe<T, N, 32, 4>(r);
     } else {
       add_row<T, N, 16>(r);
       add_col<T, N, 16, 8>(r);

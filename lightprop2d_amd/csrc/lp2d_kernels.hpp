@@ -90,14 +90,38 @@ struct PassArgs {
   const FieldOpt* fopt;
 };
 
-// Base of field b's source / destination plane.
+// Global-address-space pointers: loads and stores through them are global_* with a
+// uniform (SGPR) base and a 32-bit lane offset.  Through a generic pointer they would
+// be flat_* with a 64-bit address per lane, which also count against the LDS wait
+// counter and so stall every LDS wait behind outstanding memory traffic.
+// (The host pass of this single-source file sees the same declarations without the
+// qualifier: address spaces exist only in the device compilation.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define LP2D_GLB __attribute__((address_space(1)))
+#else
+#define LP2D_GLB
+#endif
 template <typename T>
-__device__ __forceinline__ const cx<T>* field_src(const PassArgs<T>& a, int b) {
-  return a.src_tab ? a.src_tab[b] : a.src + (long long)b * a.fstride;
+__device__ __forceinline__ const LP2D_GLB T* glb(const T* p) { return (const LP2D_GLB T*)p; }
+template <typename T>
+__device__ __forceinline__ LP2D_GLB T* glb(T* p) { return (LP2D_GLB T*)p; }
+// A pointer that is the same on every lane, moved to scalar registers.
+template <typename T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+  const uint64_t v = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (T*)(((uint64_t)hi << 32) | lo);
+}
+
+// Base of field b's source / destination plane (b uniform across the workgroup).
+template <typename T>
+__device__ __forceinline__ const LP2D_GLB cx<T>* field_src(const PassArgs<T>& a, int b) {
+  return glb(uniform_ptr(a.src_tab ? a.src_tab[b] : a.src + (long long)b * a.fstride));
 }
 template <typename T>
-__device__ __forceinline__ cx<T>* field_dst(const PassArgs<T>& a, int b) {
-  return a.dst_tab ? a.dst_tab[b] : a.dst + (long long)b * a.fstride;
+__device__ __forceinline__ LP2D_GLB cx<T>* field_dst(const PassArgs<T>& a, int b) {
+  return glb(uniform_ptr(a.dst_tab ? a.dst_tab[b] : a.dst + (long long)b * a.fstride));
 }
 // The call's optics, or field b's when the batch carries a per-field table.
 template <typename T>
@@ -409,7 +433,7 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
   const long long ubase = (long long)y0 * N;
   const unsigned lo = (unsigned)((y - y0) * N + t);
   cx<T> v[E];
-  const cx<T>* __restrict__ src = field_src(a, b) + ubase;
+  const LP2D_GLB cx<T>* __restrict__ src = field_src(a, b) + ubase;
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = src[lo + (unsigned)(TL * m)];
   if constexpr (FEAT) {
@@ -424,7 +448,7 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
   }
   LineAcc<T, PS> acc{lds + l * L};
   line_fft<T, N, E, INV>(v, t, acc, (INV ? a.tw_inv : a.tw) + (N - 2));
-  cx<T>* __restrict__ dst = field_dst(a, b) + ubase;
+  LP2D_GLB cx<T>* __restrict__ dst = field_dst(a, b) + ubase;
   if constexpr (FEAT) {
     spatial_epi_all<T, E>(a, v, b, y, t, TL, N, frow + t, valid);
     if ((a.flags & F_NO_STORE) == 0) {
@@ -522,8 +546,8 @@ __device__ __forceinline__ void cols_body(const PassArgs<T>& a, int b0, int b1, 
     const long long ubase = (long long)b * a.fstride + x0;
     cx<T> v[E];
     // the shared-factor class never runs on per-field tables (their optics differ)
-    const cx<T>* __restrict__ src = (MIDK == 3 ? a.src + (long long)b * a.fstride
-                                               : field_src(a, b)) + x0;
+    const LP2D_GLB cx<T>* __restrict__ src =
+        (MIDK == 3 ? glb(a.src + (long long)b * a.fstride) : field_src(a, b)) + x0;
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = src[lo + (unsigned)(TL * N * m)];
     if constexpr ((MODE & MODE_FWD) != 0) line_fft<T, N, E, false>(v, t, acc, a.tw + (N - 2));
@@ -555,11 +579,161 @@ __device__ __forceinline__ void cols_body(const PassArgs<T>& a, int b0, int b1, 
     asm volatile("" ::: "memory");
     if constexpr ((MODE & MODE_INV) != 0)
       line_fft<T, N, E, true>(v, t, acc, a.tw_inv + (N - 2));
-    cx<T>* __restrict__ dst = (MIDK == 3 ? a.dst + (long long)b * a.fstride : field_dst(a, b)) + x0;
+    LP2D_GLB cx<T>* __restrict__ dst =
+        (MIDK == 3 ? glb(a.dst + (long long)b * a.fstride) : field_dst(a, b)) + x0;
     const T sc = (a.flags & F_SCALE) ? a.scale : (T)1;
 #pragma unroll
     for (int m = 0; m < E; ++m) dst[lo + (unsigned)(TL * N * m)] = mk(v[m].re * sc, v[m].im * sc);
   }
+}
+
+// ----------------------------------------------------------------- pipelined columns
+// frac(Kz*z) in cycles for the complex64 path, with Kz = sqrt(max(kz2, 0)).  The square
+// root is v_rsq_f64 refined by one Newton step in FP64 (relative error ~1e-14, far
+// below what the FP32 phase resolves) instead of the correctly rounded expansion.
+// kz2 <= 0 (evanescent, Kz := 0 as beam2d.py:307-308) clamps to 1e-300: then
+// Kz*z ~ 1e-150*z and H = 1.  fract() differs from the reference's d - trunc(d) by
+// whole cycles only (for z < 0), which leave exp(-2*pi*i*fr) unchanged.
+__device__ __forceinline__ float kz_frac_fast(double kz2, double z) {
+  const double q = fmax(kz2, 1e-300);
+  const double y = __builtin_amdgcn_rsq(q);
+  double s = q * y;
+  s = fma(fma(-s, s, q), 0.5 * y, s);
+  return (float)__builtin_amdgcn_fract(s * z);
+}
+
+// Transfer function x aperture of one column, frequency fx: per-column constants.
+struct ColMid {
+  double c0;   // k0^2 - fx^2
+  double fx2;  // fx^2
+  double z;    // 0 when the call has no transfer function (then H = 1)
+  double ap;   // squared pupil radius, +inf without a pupil
+  double inv_ndL;
+};
+__device__ __forceinline__ ColMid col_mid(const FieldOpt& o, int flags, int sx) {
+#pragma clang fp contract(off)
+  ColMid m;
+  const double fx = (double)sx * o.inv_ndL;
+  m.fx2 = fx * fx;
+  m.c0 = o.k0sq - m.fx2;
+  m.z = (flags & F_H) ? o.z : 0.0;
+  m.ap = (flags & F_APERTURE) ? o.ap_r2 : __builtin_huge_val();
+  m.inv_ndL = o.inv_ndL;
+  return m;
+}
+// v * H(fy) * pupil(fy) at signed frequency index sy of the column (complex64 path,
+// evanescent modes passed through).  Branch-free: same operation order as
+// spectral_mid for Kz^2 and the pupil test.
+__device__ __forceinline__ cx<float> mid_fast(const ColMid& m, cx<float> v, int sy) {
+#pragma clang fp contract(off)
+  const double fy = (double)sy * m.inv_ndL;
+  const double fy2 = fy * fy;
+  const float fr = kz_frac_fast(m.c0 - fy2, m.z);
+  const cx<float> h = mk(__builtin_amdgcn_cosf(fr), __builtin_amdgcn_sinf(fr));
+  const bool out = m.fx2 + fy2 > m.ap;
+  const cx<float> r = cmulc(v, h);  // exp(-2*pi*i*fr)
+  return mk(out ? 0.0f : r.re, out ? 0.0f : r.im);
+}
+
+// One field of a pipelined column group, staged in LDS at kernel start so that the
+// field loop reads its pointers and optics without global loads (a global load there
+// would wait behind the prefetch and the previous field's stores).
+struct PipeField {
+  const void* src;
+  void* dst;
+  FieldOpt o;
+};
+constexpr int kPipeMaxPer = 32;
+
+template <typename T>
+__device__ __forceinline__ void pipe_stage_fields(const PassArgs<T>& a, int b0, int b1,
+                                                  PipeField* pf, int tid) {
+  if (tid < b1 - b0) {
+    const int b = b0 + tid;
+    PipeField f;
+    f.src = a.src_tab ? (const void*)a.src_tab[b] : (const void*)(a.src + (long long)b * a.fstride);
+    f.dst = a.dst_tab ? (void*)a.dst_tab[b] : (void*)(a.dst + (long long)b * a.fstride);
+    if (a.fopt) {
+      f.o = a.fopt[b];
+    } else {
+      f.o.k0sq = a.k0sq;
+      f.o.inv_ndL = a.inv_ndL;
+      f.o.z = a.z;
+      f.o.ap_r2 = a.ap_r2;
+    }
+    pf[tid] = f;
+  }
+}
+
+// Column strips of a group of fields [b0, b1) with software pipelining: the strip of
+// field b+1 is loaded into registers while field b is transformed, so the memory
+// traffic of one field overlaps the arithmetic of the previous one instead of every
+// workgroup alternating between the two in step.  Twiddles come from LDS (twl) and
+// the fields' pointers and optics from pf, so that nothing in the loop waits for the
+// prefetch before it is needed.  Column FFT -> H, pupil -> column IFFT; complex64
+// only, evanescent modes pass through (EV_PASS), no mask multipliers.
+template <typename T, int N, int E, int W, int PS>
+__device__ __forceinline__ void cols_pipe_body(const PassArgs<T>& a, int b0, int b1, int x0,
+                                               cx<T>* lds, const cx<T>* twl,
+                                               const PipeField* pf, int tid) {
+  static_assert(sizeof(T) == 4, "complex64 path");
+  constexpr int TL = N / E;
+  const int c = tid % W, t = tid / W;
+  const int x = x0 + c;
+  const int sx = x < N / 2 ? x : x - N;
+  const unsigned lo = (unsigned)(t * N + c);
+  InterAcc<T, PS, W> acc{lds + c};
+  const int flags = a.flags;
+  const bool mid = (flags & (F_H | F_APERTURE)) != 0;
+  const bool spec_out = (flags & F_SPEC_OUT) != 0;
+  const bool scale = (flags & F_SCALE) != 0;
+  // Step b moves field b's strip (prefetched by step b-1) from nxt into cur, requests
+  // field b+1 into nxt, then transforms cur and stores it.  The copy at the head of
+  // step b+1 therefore has exactly b's stores behind the loads it waits for, and the
+  // compiler's wait count lets those stores stay in flight.  The first step is peeled
+  // off the loop so that the loop head is only ever reached in that state (merged
+  // with the loop entry the wait would cover the stores as well).
+  cx<T> cur[E], nxt[E];
+  auto fetch = [&](int bf) {
+    const LP2D_GLB cx<T>* __restrict__ src = glb(uniform_ptr((const cx<T>*)pf[bf - b0].src)) + x0;
+#pragma unroll
+    for (int m = 0; m < E; ++m) nxt[m] = src[lo + (unsigned)(TL * N * m)];
+  };
+  auto step = [&](int b) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) cur[m] = nxt[m];
+    // the last step re-reads its own strip (still in L2): one uniform path, no
+    // branch around the loads
+    fetch(b + 1 < b1 ? b + 1 : b);
+    line_fft_l<T, N, E, false>(cur, t, acc, twl);
+    if (mid) {
+      ColMid cm = col_mid(pf[b - b0].o, flags, sx);
+      // opaque per field: otherwise the per-element frequencies (the same for every
+      // field of a strip) are hoisted out of the field loop and held for all E
+      // elements at once
+      int tt = t;
+      asm volatile("" : "+v"(cm.c0), "+v"(cm.fx2), "+v"(cm.inv_ndL), "+v"(tt));
+      // fftfreq order: index t + TL*m is a negative frequency for m >= E/2
+#pragma unroll
+      for (int m = 0; m < E; ++m) cur[m] = mid_fast(cm, cur[m], tt + TL * m - (m >= E / 2 ? N : 0));
+    }
+    if (spec_out) {
+      LP2D_GLB cx<T>* __restrict__ so = glb(a.spec_out + (long long)b * a.fstride + x0);
+#pragma unroll
+      for (int m = 0; m < E; ++m) so[lo + (unsigned)(TL * N * m)] = cur[m];
+    }
+    line_fft_l<T, N, E, true>(cur, t, acc, twl);
+    if (scale) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) cur[m] = cscale(cur[m], a.scale);
+    }
+    LP2D_GLB cx<T>* __restrict__ dst = glb(uniform_ptr((cx<T>*)pf[b - b0].dst)) + x0;
+#pragma unroll
+    for (int m = 0; m < E; ++m) dst[lo + (unsigned)(TL * N * m)] = cur[m];
+  };
+  fetch(b0);
+  step(b0);
+  for (int b = b0 + 1; b < b1; ++b) step(b);
 }
 
 // ----------------------------------------------------------------- whole field in LDS
@@ -579,13 +753,13 @@ __device__ __forceinline__ void small_body(const PassArgs<T>& a, int b, int mode
   const cx<T>* tw = a.tw + (N - 2);
   const cx<T>* twi = a.tw_inv + (N - 2);
   const long long fb = (long long)b * a.fstride;
-  const cx<T>* fsrc = field_src(a, b);
-  cx<T>* fdst = field_dst(a, b);
+  const LP2D_GLB cx<T>* fsrc = field_src(a, b);
+  LP2D_GLB cx<T>* fdst = field_dst(a, b);
   cx<T> v[E];
   // phase 1: rows (load, spatial prologue, forward row FFT)
   {
     const int y = tid / TL, t = tid % TL;
-    const cx<T>* __restrict__ src = fsrc + (long long)y * N;
+    const LP2D_GLB cx<T>* __restrict__ src = fsrc + (long long)y * N;
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = src[t + TL * m];
     if (a.flags & F_SPATIAL) {
@@ -634,7 +808,7 @@ __device__ __forceinline__ void small_body(const PassArgs<T>& a, int b, int mode
       __syncthreads();
       line_fft<T, N, E, true>(v, t, acc, twi);
     }
-    cx<T>* __restrict__ dst = fdst + (long long)y * N;
+    LP2D_GLB cx<T>* __restrict__ dst = fdst + (long long)y * N;
     spatial_epi_all<T, E>(a, v, b, y, t, TL, N, fb + (long long)y * N + t, true);
     if ((a.flags & F_NO_STORE) == 0) {
 #pragma unroll

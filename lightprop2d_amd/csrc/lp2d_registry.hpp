@@ -29,7 +29,8 @@ template <typename T>
 struct ColVar {
   ColLaunch<T> fn;
   int E, W, threads, lds;
-  bool hoist;  // transfer function computed once per group of fields
+  int kind;  // bit 0: transfer function computed once per group of fields (shared-H)
+             // bit 1: software-pipelined over the fields of a group
 };
 template <typename T>
 struct SmallVar {
@@ -37,7 +38,7 @@ struct SmallVar {
   int E, threads, lds;
 };
 
-constexpr int kMaxVariants = 8;
+constexpr int kMaxVariants = 12;
 constexpr int kLogMax = 13;  // n <= 8192
 
 template <typename T>
@@ -49,6 +50,9 @@ struct Registry {
   SmallVar<T> small[kLogMax + 1];
   bool has_small[kLogMax + 1];
 };
+
+// Workgroups of a column-pass kernel resident at once on the current device.
+int resident_groups(const void* kernel, int threads);
 
 void register_f32_a(Registry<float>& r);   // n = 2 .. 256
 void register_f32_b(Registry<float>& r);   // n = 512, 1024
