@@ -29,8 +29,8 @@ METRIC = ("field-propagations/s at 1024² c64 (fwd & fwd+adjoint), "
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=50)
-    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--batch", type=int, default=32)
     p.add_argument("--n", type=int, default=1024)
     p.add_argument("--cpu-seconds", type=float, default=12.0,
@@ -160,7 +160,7 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                                    "(batches of independent fields shard by field with no "
-                                   "communication; see DESIGN.md)"}))
+                                   "communication)"}))
         sys.exit(2)
     import torch
     from lightprop2d_amd import _lib as L

@@ -300,6 +300,36 @@ int run_small(const PassArgs<T>& a, int lg, int mode, hipStream_t s) {
   return e == hipSuccess ? LP2D_OK : hip_fail(e, "whole-field launch");
 }
 
+// The gradient pass carries the screen factor and the accumulators: it runs the row
+// schedule with the fewest elements per thread (lowest register pressure).
+template <typename T>
+const RowVar<T>& grad_variant(int lg) {
+  int vi = 0;
+  for (int i = 1; i < reg<T>()->nrows[lg]; ++i)
+    if (reg<T>()->rows[lg][i].E < reg<T>()->rows[lg][vi].E) vi = i;
+  return reg<T>()->rows[lg][vi];
+}
+
+// F_TILED when the selected schedules can hand the intermediate plane between the
+// sweeps over in the block-tiled layout: the pipelined complex64 column class with
+// strips of kTileCols, row tiles of whole kTileRows-row blocks, and (grad) the
+// adjoint's gradient pass reading it (whole blocks again, unless it stores nothing).
+template <typename T>
+int tiled_flag(const PassArgs<T>& cols_a, int lg, bool grad = false, bool grad_store = true) {
+  if (sizeof(T) != 4) return 0;
+  const int n = 1 << lg;
+  const ColVar<T>& cv = reg<T>()->cols[lg][g_sel[dt<T>()][1][lg]];
+  const RowVar<T>& rv = reg<T>()->rows[lg][g_sel[dt<T>()][0][lg]];
+  if (!(cv.kind & 2) || cv.W != kTileCols || (n / cv.E) % kTileRows != 0) return 0;
+  if (!pipe_eligible(cols_a, MODE_FWD | MODE_INV)) return 0;
+  if (rv.lines % kTileRows != 0 || (n / rv.E) % kTileCols != 0) return 0;
+  if (grad) {
+    const RowVar<T>& gv = grad_variant<T>(lg);
+    if ((gv.grad_lines % kTileRows != 0 && grad_store) || (n / gv.E) % kTileCols != 0) return 0;
+  }
+  return F_TILED;
+}
+
 // ----------------------------------------------------------------- elementwise kernels
 template <typename T>
 __global__ void k_spatial_only(PassArgs<T> a, int n, long long total) {
@@ -612,27 +642,25 @@ int op_propagate_chunk(int n, int lg, int batch, const void* in, void* out, cons
     a.scale = sc;
     return run_small<T>(a, lg, MODE_FWD | MODE_INV, s);
   }
-  a.flags = pro;
+  a.flags = mid;
+  const int tl = tiled_flag<T>(a, lg);
+  a.flags = pro | tl;
   if (int rc = run_rows<T>(a, lg, false, s)) return rc;
   a.src = (const cx<T>*)out;
-  a.flags = mid;
+  a.flags = mid | tl;
   if (int rc = run_cols<T>(a, lg, MODE_FWD | MODE_INV, s)) return rc;
-  a.flags = F_SCALE;
+  a.flags = F_SCALE | tl;
   a.scale = sc;
   return run_rows<T>(a, lg, true, s);
 }
 
 template <typename T>
 int run_rows_grad(const PassArgs<T>& a, int lg, float* gphase, hipStream_t s) {
-  // the gradient pass carries the screen factor and the accumulators: use the
-  // schedule with the fewest elements per thread (lowest register pressure)
-  int vi = 0;
-  for (int i = 1; i < reg<T>()->nrows[lg]; ++i)
-    if (reg<T>()->rows[lg][i].E < reg<T>()->rows[lg][vi].E) vi = i;
-  const RowVar<T>& v = reg<T>()->rows[lg][vi];
+  const RowVar<T>& v = grad_variant<T>(lg);
   const int n = 1 << lg;
-  const int tpf = (n + v.lines - 1) / v.lines;
-  int groups = 2048 / tpf;
+  const int tpf = (n + v.grad_lines - 1) / v.grad_lines;
+  // about 512K threads in flight, whatever the tile height
+  int groups = (2048 * 256 / ((n / v.E) * v.grad_lines)) / tpf;
   if (groups < 1) groups = 1;
   if (groups > a.batch) groups = a.batch;
   std::string nm;
@@ -685,16 +713,18 @@ int op_adjoint(int n, int lg, int batch, const void* gout, const void* u, const 
     a.flags = mid | epi;
     if (int rc = run_small<T>(a, lg, MODE_FWD | MODE_INV, s)) return rc;
   } else {
-    a.flags = 0;
+    a.flags = mid;
+    const int tl = tiled_flag<T>(a, lg, shared_sum, no_store == 0);
+    a.flags = tl;
     if (int rc = run_rows<T>(a, lg, false, s)) return rc;
     a.src = (const cx<T>*)gin;
-    a.flags = mid;
+    a.flags = mid | tl;
     if (int rc = run_cols<T>(a, lg, MODE_FWD | MODE_INV, s)) return rc;
     if (shared_sum) {
-      a.flags = F_SCALE | no_store;
+      a.flags = F_SCALE | no_store | tl;
       return run_rows_grad<T>(a, lg, gphase, s);
     }
-    a.flags = epi;
+    a.flags = epi | tl;
     if (int rc = run_rows<T>(a, lg, true, s)) return rc;
   }
   if (shared_sum) {  // per-field planes from the whole-field kernel: reduce into plane 0
@@ -733,12 +763,14 @@ int run_bucket(const Bucket& bk, hipStream_t s) {
     a.scale = sc;
     return run_small<T>(a, bk.lg, MODE_FWD | MODE_INV, s);
   }
-  a.flags = 0;
+  a.flags = bk.flags;
+  const int tl = tiled_flag<T>(a, bk.lg);
+  a.flags = tl;
   if (int rc = run_rows<T>(a, bk.lg, false, s)) return rc;
   a.src_tab = (const cx<T>* const*)bk.d_dst;  // later sweeps work in the destinations
-  a.flags = bk.flags;
+  a.flags = bk.flags | tl;
   if (int rc = run_cols<T>(a, bk.lg, MODE_FWD | MODE_INV, s)) return rc;
-  a.flags = F_SCALE;
+  a.flags = F_SCALE | tl;
   a.scale = sc;
   return run_rows<T>(a, bk.lg, true, s);
 }

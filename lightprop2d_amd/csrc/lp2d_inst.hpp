@@ -118,6 +118,8 @@ struct SmallShape {
 // ----------------------------------------------------------------- launchers
 template <typename T, int N, int E, int LINES, int PS>
 hipError_t launch_rows(const PassArgs<T>& a, bool inv, hipStream_t s) {
+  if ((a.flags & F_TILED) && (LINES % kTileRows != 0 || (N / E) % kTileCols != 0))
+    return hipErrorInvalidValue;
   constexpr int threads = (N / E) * LINES;
   constexpr int TPF = (N + LINES - 1) / LINES;
   const unsigned grid = (unsigned)((long long)a.batch * TPF);
@@ -138,6 +140,10 @@ hipError_t launch_rows(const PassArgs<T>& a, bool inv, hipStream_t s) {
 
 template <typename T, int N, int E, int LINES, int PS>
 hipError_t launch_rows_grad(const PassArgs<T>& a, int groups, float* partial, hipStream_t s) {
+  // reading a tiled plane in place needs whole 8-row blocks unless nothing is stored
+  if ((a.flags & F_TILED) &&
+      (((LINES % kTileRows) != 0 && !(a.flags & F_NO_STORE)) || (N / E) % kTileCols != 0))
+    return hipErrorInvalidValue;
   constexpr int threads = (N / E) * LINES;
   constexpr int TPF = (N + LINES - 1) / LINES;
   k_rows_grad<T, N, E, LINES, PS><<<(unsigned)(TPF * groups), threads, 0, s>>>(a, groups, partial);
@@ -150,6 +156,7 @@ constexpr int kColBlocks = 1024;
 
 template <typename T, int N, int E, int W, int PS, bool HOIST>
 hipError_t launch_cols(const PassArgs<T>& a, int mode, hipStream_t s) {
+  if (a.flags & F_TILED) return hipErrorInvalidValue;  // natural layout only
   constexpr int threads = (N / E) * W;
   constexpr int SPF = N / W;
   // the shared-factor class needs one set of optics for the whole call
@@ -209,9 +216,11 @@ hipError_t launch_cols(const PassArgs<T>& a, int mode, hipStream_t s) {
 // round of resident workgroups.
 template <typename T, int N, int E, int W, int PS>
 hipError_t launch_cols_pipe(const PassArgs<T>& a, int mode, hipStream_t s) {
-  constexpr int FI = MODE_FWD | MODE_INV;
-  const bool ok = sizeof(T) == 4 && mode == FI && a.evan == EV_PASS &&
-                  !(a.flags & (F_SMASK0 | F_SMASK1));
+  const bool ok = pipe_eligible(a, mode);
+  // the block-tiled layout is only defined for strips of kTileCols columns, and only
+  // this class reads it
+  if ((a.flags & F_TILED) && (!ok || W != kTileCols || (N / E) % kTileRows != 0))
+    return hipErrorInvalidValue;
   if constexpr (sizeof(T) == 4) {
     if (ok) {
       constexpr int threads = (N / E) * W;
@@ -244,9 +253,11 @@ void add_row(Registry<T>& r) {
   constexpr int lg = ilog2c(N);
   int& k = r.nrows[lg];
   if (k >= kMaxVariants) return;
+  // (taller gradient tiles, for reading a block-tiled plane in place, measured slower:
+  // a 1024-thread workgroup per CU stalls the CU at every exchange barrier)
   r.rows[lg][k++] = RowVar<T>{&launch_rows<T, N, E, S::LINES, S::PS>,
                               &launch_rows_grad<T, N, E, S::LINES, S::PS>, E, S::LINES,
-                              S::THREADS, S::LDS};
+                              S::THREADS, S::LDS, S::LINES};
 }
 
 template <typename T, int N, int E, int WT, bool HOIST = false>

@@ -33,7 +33,17 @@ enum : int {
   F_INTENSITY = 1 << 11, //           write |v|^2
   F_GRAD = 1 << 12,      //           adjoint phase-mask gradient
   F_NO_STORE = 1 << 13,  //           skip the dst store (gradient-only epilogue)
+  F_TILED = 1 << 14,     // the intermediate plane between the sweeps is block-tiled
 };
+
+// Block-tiled intermediate plane (F_TILED): rows in blocks of 8, each block stored as
+// 8-column strips of 8 x 8 elements, i.e. element (y, x) of an n x n plane at
+//   (y/8)*8n + (x/8)*64 + (y%8)*8 + x%8.
+// A block of 8 rows occupies the same addresses as in row-major order, so a row pass
+// over whole blocks converts in place; a column strip of width 8 is then 512-byte
+// chunks instead of 64-byte pieces of 1024 different rows, and every 128-byte line is
+// written whole by one store instruction.
+constexpr int kTileRows = 8, kTileCols = 8;
 constexpr int F_SPATIAL = F_MASK0 | F_MASK1 | F_LENS | F_PHASE | F_FIELD_OUT;
 constexpr int F_MID = F_H | F_APERTURE | F_SMASK0 | F_SMASK1 | F_SPEC_OUT;
 
@@ -432,10 +442,19 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
   // uniform base (SGPRs) + one 32-bit per-lane offset
   const long long ubase = (long long)y0 * N;
   const unsigned lo = (unsigned)((y - y0) * N + t);
+  // block-tiled offset of (y, x = t + TL*m): lo_t + 8*TL*m (y0 a multiple of 8, TL of 8)
+  const bool tiled = (a.flags & F_TILED) != 0;
+  const unsigned lo_t = (unsigned)((l >> 3) * kTileRows * N + (t >> 3) * (kTileRows * kTileCols) +
+                                   (l & 7) * kTileCols + (t & 7));
   cx<T> v[E];
   const LP2D_GLB cx<T>* __restrict__ src = field_src(a, b) + ubase;
+  if (INV && tiled) {
 #pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = src[lo + (unsigned)(TL * m)];
+    for (int m = 0; m < E; ++m) v[m] = src[lo_t + (unsigned)(kTileRows * TL * m)];
+  } else {
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = src[lo + (unsigned)(TL * m)];
+  }
   if constexpr (FEAT) {
     if (a.flags & F_SPATIAL) {
       spatial_pro_all<T, E>(a, v, b, y, t, TL, N);
@@ -452,15 +471,26 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
   if constexpr (FEAT) {
     spatial_epi_all<T, E>(a, v, b, y, t, TL, N, frow + t, valid);
     if ((a.flags & F_NO_STORE) == 0) {
+      if (!INV && tiled) {
 #pragma unroll
-      for (int m = 0; m < E; ++m)
-        if (valid) dst[lo + (unsigned)(TL * m)] = v[m];
+        for (int m = 0; m < E; ++m) dst[lo_t + (unsigned)(kTileRows * TL * m)] = v[m];
+      } else {
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+          if (valid) dst[lo + (unsigned)(TL * m)] = v[m];
+      }
     }
   } else {
     const T sc = (a.flags & F_SCALE) ? a.scale : (T)1;
+    if (!INV && tiled) {
 #pragma unroll
-    for (int m = 0; m < E; ++m)
-      if (valid) dst[lo + (unsigned)(TL * m)] = mk(v[m].re * sc, v[m].im * sc);
+      for (int m = 0; m < E; ++m)
+        dst[lo_t + (unsigned)(kTileRows * TL * m)] = mk(v[m].re * sc, v[m].im * sc);
+    } else {
+#pragma unroll
+      for (int m = 0; m < E; ++m)
+        if (valid) dst[lo + (unsigned)(TL * m)] = mk(v[m].re * sc, v[m].im * sc);
+    }
   }
 }
 
@@ -487,11 +517,22 @@ __device__ __forceinline__ void rows_grad_body(const PassArgs<T>& a, int b0, int
     ec[m] = grad_fold(a, mk(a.scale, (T)0), 0, y, t + TL * m, N);
     gsum[m] = 0.f;
   }
+  // block-tiled offset of (y, x = t + TL*m) in the field: lo_t + 8*TL*m (TL a multiple
+  // of 8); any row tile height can read it, storing in place needs whole blocks
+  const bool tiled = (a.flags & F_TILED) != 0;
+  const unsigned lo_t = (unsigned)((y >> 3) * kTileRows * N + (t >> 3) * (kTileRows * kTileCols) +
+                                   (y & 7) * kTileCols + (t & 7));
   for (int b = b0; b < b1; ++b) {
     const long long frow = (long long)b * a.fstride + (long long)y * N;
     cx<T> v[E];
+    if (tiled) {
+      const LP2D_GLB cx<T>* __restrict__ src = glb(a.src + (long long)b * a.fstride);
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = a.src[frow + t + TL * m];
+      for (int m = 0; m < E; ++m) v[m] = src[lo_t + (unsigned)(kTileRows * TL * m)];
+    } else {
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = a.src[frow + t + TL * m];
+    }
     line_fft<T, N, E, true>(v, t, acc, a.tw_inv + (N - 2));
     const cx<T>* __restrict__ us = a.u_saved + (long long)b * a.u_sb + (long long)y * N;
 #pragma unroll
@@ -672,6 +713,14 @@ __device__ __forceinline__ void pipe_stage_fields(const PassArgs<T>& a, int b0, 
 // the fields' pointers and optics from pf, so that nothing in the loop waits for the
 // prefetch before it is needed.  Column FFT -> H, pupil -> column IFFT; complex64
 // only, evanescent modes pass through (EV_PASS), no mask multipliers.
+// Calls the pipelined class takes (host side): the fused forward+inverse pass of the
+// complex64 path without mask multipliers, evanescent modes passed through.
+template <typename T>
+inline bool pipe_eligible(const PassArgs<T>& a, int mode) {
+  return sizeof(T) == 4 && mode == (MODE_FWD | MODE_INV) && a.evan == EV_PASS &&
+         !(a.flags & (F_SMASK0 | F_SMASK1));
+}
+
 template <typename T, int N, int E, int W, int PS>
 __device__ __forceinline__ void cols_pipe_body(const PassArgs<T>& a, int b0, int b1, int x0,
                                                cx<T>* lds, const cx<T>* twl,
@@ -681,7 +730,13 @@ __device__ __forceinline__ void cols_pipe_body(const PassArgs<T>& a, int b0, int
   const int c = tid % W, t = tid / W;
   const int x = x0 + c;
   const int sx = x < N / 2 ? x : x - N;
-  const unsigned lo = (unsigned)(t * N + c);
+  // natural: (y, x) at y*N + x; block-tiled (W == kTileCols): see F_TILED.  Row
+  // y = t + TL*m is at lo + TL*N*m in both (TL a multiple of 8 when tiled).
+  const bool tiled = (a.flags & F_TILED) != 0;
+  const unsigned lo_n = (unsigned)(t * N + c);
+  const unsigned lo =
+      tiled ? (unsigned)((t >> 3) * kTileRows * N + (t & 7) * kTileCols + c) : lo_n;
+  const int xoff = tiled ? (x0 / kTileCols) * (kTileRows * kTileCols) : x0;
   InterAcc<T, PS, W> acc{lds + c};
   const int flags = a.flags;
   const bool mid = (flags & (F_H | F_APERTURE)) != 0;
@@ -695,7 +750,7 @@ __device__ __forceinline__ void cols_pipe_body(const PassArgs<T>& a, int b0, int
   // with the loop entry the wait would cover the stores as well).
   cx<T> cur[E], nxt[E];
   auto fetch = [&](int bf) {
-    const LP2D_GLB cx<T>* __restrict__ src = glb(uniform_ptr((const cx<T>*)pf[bf - b0].src)) + x0;
+    const LP2D_GLB cx<T>* __restrict__ src = glb(uniform_ptr((const cx<T>*)pf[bf - b0].src)) + xoff;
 #pragma unroll
     for (int m = 0; m < E; ++m) nxt[m] = src[lo + (unsigned)(TL * N * m)];
   };
@@ -720,14 +775,14 @@ __device__ __forceinline__ void cols_pipe_body(const PassArgs<T>& a, int b0, int
     if (spec_out) {
       LP2D_GLB cx<T>* __restrict__ so = glb(a.spec_out + (long long)b * a.fstride + x0);
 #pragma unroll
-      for (int m = 0; m < E; ++m) so[lo + (unsigned)(TL * N * m)] = cur[m];
+      for (int m = 0; m < E; ++m) so[lo_n + (unsigned)(TL * N * m)] = cur[m];
     }
     line_fft_l<T, N, E, true>(cur, t, acc, twl);
     if (scale) {
 #pragma unroll
       for (int m = 0; m < E; ++m) cur[m] = cscale(cur[m], a.scale);
     }
-    LP2D_GLB cx<T>* __restrict__ dst = glb(uniform_ptr((cx<T>*)pf[b - b0].dst)) + x0;
+    LP2D_GLB cx<T>* __restrict__ dst = glb(uniform_ptr((cx<T>*)pf[b - b0].dst)) + xoff;
 #pragma unroll
     for (int m = 0; m < E; ++m) dst[lo + (unsigned)(TL * N * m)] = cur[m];
   };

@@ -24,6 +24,7 @@ struct RowVar {
   RowLaunch<T> fn;
   RowGradLaunch<T> grad_fn;  // adjoint last pass with the batch-summed phase gradient
   int E, lines, threads, lds;
+  int grad_lines;  // row tile of grad_fn: whole 8-row blocks where the threads allow
 };
 template <typename T>
 struct ColVar {
