@@ -152,7 +152,45 @@ def extra_legs(torch, P, L, lib, dev):
                     "in place",
         "value": round(8 * 10 / el, 1), "unit": "field-propagations/s",
         "ms_per_chain": round(el * 1e3, 4)}
+    out["generators"] = generator_leg(torch, dev)
     return out
+
+
+def generator_leg(torch, dev):
+    """Profile generators at 1024^2 on the Beam2D grid (area 400 um): the device path
+    (lp2d_profile; the random kind uploads only its host draw) against the same
+    generator evaluated with numpy on the host and copied to the device."""
+    from lightprop2d_amd import profiles as PP
+    n, dL = 1024, 400e-4 / 1024
+    xh = np.arange(-n // 2, n // 2, 1) * dL
+    yh = xh.reshape((-1, 1))
+    xd = torch.from_numpy(xh).to(dev)
+    yd = xd.reshape((-1, 1))
+    res = {}
+    for name, args in (("gaussian_beam", (1.0, 25e-4)), ("round_hole", (25e-4,)),
+                       ("random_round_hole_phase", (25e-4,))):
+        fn = getattr(PP, name)
+
+        def device_call():
+            return fn(xd.clone(), yd.clone(), *args)
+
+        def host_call():
+            return torch.from_numpy(np.asarray(fn(xh.copy(), yh.copy(), *args))).to(dev)
+
+        row = {}
+        for label, call in (("device_us", device_call), ("host_numpy_plus_copy_us", host_call)):
+            call()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(10):
+                call()
+            torch.cuda.synchronize()
+            row[label] = round((time.perf_counter() - t0) / 10 * 1e6, 1)
+        res[name] = row
+    res["workload"] = ("one 1024x1024 generator call on the Beam2D grid (area 400 um), "
+                       "result resident on the device; random draws from numpy's RNG on "
+                       "the host in both paths")
+    return res
 
 
 def main():

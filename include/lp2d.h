@@ -122,6 +122,40 @@ int lp2d_intensity(int64_t count, int dtype, const void* field, void* out, void*
 int lp2d_moments(int n, int batch, int dtype, const void* field, double dL, double* out,
                  void* stream);
 
+/* ---------------------------------------------------------------- generators */
+/* Field / mask generators of lightprop2d/profiles.py evaluated on the device over the
+ * broadcast grid (y[j], x[i]), out[j*nx + i] (the reference calls them as
+ * f(X, Y) with X of shape (n,) and Y of shape (n, 1), beam2d.py:302, 338).
+ *   LP2D_PROF_PLANE        1                                 (profiles.py:35-38)
+ *   LP2D_PROF_GAUSSIAN     a0*exp(-((x-x0)^2 + (y-y0)^2)/rho0^2)       (:64-71)
+ *   LP2D_PROF_ROUND_HOLE   exp(-((x-x0)^2+(y-y0)^2)/r^2) >= thresh     (:74-81)
+ *   LP2D_PROF_RECT_HOLE    |x-x0| < dx/2 and |y-y0| < dy/2             (:116-128)
+ *   LP2D_PROF_SLITS        square holes of side dx at x0 -+ slits/2    (:131-136)
+ *   LP2D_PROF_RANDOM       kron(rand, ones(b, b))                      (:29-61)
+ *   LP2D_PROF_RANDOM_HOLE  kron(rand, ones(b, b)) * round hole         (:84-101)
+ *   LP2D_PROF_RANDOM_PHASE round hole * exp(2*pi*i * kron(rand) * hole) (:104-113)
+ * The random kinds take `rand`, a device array of (ny/b) x (nx/b) doubles drawn on
+ * the host (the reference's numpy RNG stream); the others ignore it.  thresh is the
+ * caller's 1/exp(0.5) so the comparison uses the reference's own constant.
+ * out_dtype: LP2D_MASK_U8 (bool), LP2D_MASK_F64 or LP2D_MASK_C128. */
+#define LP2D_PROF_PLANE 0
+#define LP2D_PROF_GAUSSIAN 1
+#define LP2D_PROF_ROUND_HOLE 2
+#define LP2D_PROF_RECT_HOLE 3
+#define LP2D_PROF_SLITS 4
+#define LP2D_PROF_RANDOM 5
+#define LP2D_PROF_RANDOM_HOLE 6
+#define LP2D_PROF_RANDOM_PHASE 7
+typedef struct lp2d_profile_params {
+  double a0, rho0, r, thresh;
+  double x0, y0, dx, dy, slits;
+  int32_t binning;
+  int32_t reserved;
+} lp2d_profile_params;
+int lp2d_profile(int kind, int nx, int ny, const double* x, const double* y,
+                 const lp2d_profile_params* params, const double* rand, void* out,
+                 int out_dtype, void* stream);
+
 /* ---------------------------------------------------------------- field -> field */
 /* field_out = ifft2(H(z) * aperture * fft2(field_in * exp(i*phase) * lens)), fused;
  * spectrum_out (may be NULL) receives H*aperture*fft2(...), i.e. the reference's

@@ -40,7 +40,13 @@ EXPORTED = [
     "lp2d_batch_run", "lp2d_batch_destroy", "lp2d_variant_count",
     "lp2d_set_variant", "lp2d_get_variant", "lp2d_variant_name", "lp2d_trace_enable",
     "lp2d_trace_collect", "lp2d_moments", "lp2d_set_chunk", "lp2d_get_chunk",
+    "lp2d_profile",
 ]
+
+# generator kinds of lp2d_profile
+PROF = {"plane": 0, "gaussian": 1, "round_hole": 2, "rect_hole": 3, "slits": 4,
+        "random": 5, "random_hole": 6, "random_phase": 7}
+LP2D_MASK_U8, LP2D_MASK_F64, LP2D_MASK_C128 = 1, 3, 5
 
 
 class Lp2dError(RuntimeError):
@@ -66,6 +72,13 @@ class lp2d_field_desc(C.Structure):
     _fields_ = [("n", C.c_int32), ("reserved", C.c_int32), ("src", C.c_void_p),
                 ("dst", C.c_void_p), ("area_size", C.c_double), ("wl", C.c_double),
                 ("z", C.c_double), ("aperture_radius", C.c_double)]
+
+
+class lp2d_profile_params(C.Structure):
+    _fields_ = [("a0", C.c_double), ("rho0", C.c_double), ("r", C.c_double),
+                ("thresh", C.c_double), ("x0", C.c_double), ("y0", C.c_double),
+                ("dx", C.c_double), ("dy", C.c_double), ("slits", C.c_double),
+                ("binning", C.c_int32), ("reserved", C.c_int32)]
 
 
 class lp2d_trace_stat(C.Structure):
@@ -108,6 +121,7 @@ _SIGS = {
     "lp2d_set_chunk": (_I, [_I, _I, _I]),
     "lp2d_get_chunk": (_I, [_I, _I]),
     "lp2d_trace_collect": (_I, [C.POINTER(lp2d_trace_stat), _I, C.POINTER(_I)]),
+    "lp2d_profile": (_I, [_I, _I, _I, _P, _P, C.POINTER(lp2d_profile_params), _P, _P, _I, _P]),
 }
 
 

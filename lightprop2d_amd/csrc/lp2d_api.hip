@@ -1126,3 +1126,8 @@ int lp2d_trace_collect(lp2d_trace_stat* stats, int max_stats, int* n_stats) {
 }
 
 }  // extern "C"
+
+namespace lp2d {
+// Status + message for entry points defined in other translation units.
+int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
+}  // namespace lp2d

@@ -757,9 +757,7 @@ __device__ __forceinline__ void cols_pipe_body(const PassArgs<T>& a, int b0, int
   auto step = [&](int b) {
 #pragma unroll
     for (int m = 0; m < E; ++m) cur[m] = nxt[m];
-    // the last step re-reads its own strip (still in L2): one uniform path, no
-    // branch around the loads
-    fetch(b + 1 < b1 ? b + 1 : b);
+    if (b + 1 < b1) fetch(b + 1);  // uniform branch
     line_fft_l<T, N, E, false>(cur, t, acc, twl);
     if (mid) {
       ColMid cm = col_mid(pf[b - b0].o, flags, sx);
