@@ -65,6 +65,32 @@ def cpu_baseline(n, budget_s):
                       f"in {el:.1f} s", "omp_num_threads": threads}
 
 
+def cpu_baseline_threaded(n, budget_s):
+    """The same field-propagation with scipy.fft on all host threads available to the
+    process (the reference's pyfftw ``threads=-1`` branch is not installed here)."""
+    import oracle.angular_spectrum as OA
+    try:
+        workers = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        workers = os.cpu_count() or 1
+    workers = max(1, min(workers, 16))
+    rs = np.random.RandomState(0)
+    area, wl, z = 400e-4, 632e-7, 50e-4
+    u = (rs.standard_normal((n, n)) + 1j * rs.standard_normal((n, n))).astype(np.complex64)
+    OA.propagate_field_threaded(u, area, wl, z, workers)
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        OA.propagate_field_threaded(u, area, wl, z, workers)
+        k += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or k >= 400:
+            break
+    return {"value": k / el, "unit": "field-propagations/s", "cores": workers, "kind": "port",
+            "sample": f"{k} x {n}x{n} complex128 field-propagations (scipy.fft, {workers} "
+                      f"workers; grid + H rebuilt per call) in {el:.1f} s"}
+
+
 def profiled_traffic(kernel_name):
     """HBM bytes per launch of `kernel_name` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary (profiles/hbm_traffic_propagate_1024_c64.md), or None."""
@@ -153,7 +179,40 @@ def extra_legs(torch, P, L, lib, dev):
         "value": round(8 * 10 / el, 1), "unit": "field-propagations/s",
         "ms_per_chain": round(el * 1e3, 4)}
     out["generators"] = generator_leg(torch, dev)
+    out["resample"] = resample_leg(torch, P, dev)
     return out
+
+
+def resample_leg(torch, P, dev):
+    """Beam2D.expand / coarse kernels on 8 fields of complex64: 1024^2 -> 2048^2 (edge
+    replication) and 2048^2 -> 1024^2 (2x2 block mean).  Algorithmic bytes = one read
+    of the input plus one write of the output; the numpy equivalent of one field is
+    timed beside it."""
+    B, n = 8, 1024
+    g = torch.Generator(device="cpu").manual_seed(3)
+    u = torch.randn((B, n, n), dtype=torch.complex64, generator=g).to(dev)
+    big = P.expand(u, 2 * n)
+    res = {}
+    for name, call, alg in (("expand_1024_to_2048", lambda: P.expand(u, 2 * n),
+                             8 * B * (n * n + 4 * n * n)),
+                            ("coarse_2048_to_1024", lambda: P.coarse(big, 2),
+                             8 * B * (4 * n * n + n * n))):
+        call()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(20):
+            call()
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / 20
+        res[name] = {"us_per_call": round(el * 1e6, 1), "GBs": round(alg / el / 1e9, 1),
+                     "hbm_frac": round(alg / el / 1e9 / HBM_PEAK_GBS, 4)}
+    f = np.random.RandomState(0).standard_normal((n, n)).astype(np.complex64)
+    t0 = time.perf_counter()
+    fb = np.pad(f, n // 2, mode="edge")
+    fb.reshape((n, 2, n, 2)).mean((1, 3))
+    res["numpy_one_field_us"] = round((time.perf_counter() - t0) * 1e6, 1)
+    res["workload"] = "8 fields 1024^2 complex64, expand x2 and coarse(2); device-resident"
+    return res
 
 
 def generator_leg(torch, dev):
@@ -287,9 +346,10 @@ def main():
 
     extra = {} if args.no_extra else extra_legs(torch, P, L, lib, dev)
 
-    cpu = None
+    cpu = cpu_mt = None
     if not args.no_cpu:
         cpu = cpu_baseline(n, args.cpu_seconds)
+        cpu_mt = cpu_baseline_threaded(n, min(args.cpu_seconds, 6.0))
 
     line = {
         "metric": METRIC,
@@ -317,6 +377,7 @@ def main():
         "kernels_fwd_adjoint": {k: {"launches": c, "avg_us": round(ms * 1e3 / c, 2)}
                                 for k, (c, ms) in kt2.items()},
         "cpu_baseline": cpu,
+        "cpu_baseline_threaded": cpu_mt,
         "workloads": extra,
     }
     print(json.dumps(line))

@@ -156,6 +156,17 @@ int lp2d_profile(int kind, int nx, int ny, const double* x, const double* y,
                  const lp2d_profile_params* params, const double* rand, void* out,
                  int out_dtype, void* stream);
 
+/* ---------------------------------------------------------------- resampling */
+/* Beam2D.expand (beam2d.py:375-404): (batch, n, n) -> (batch, n_out, n_out) with the
+ * field's edge rows/columns replicated (n_out - n even, d = (n_out - n)/2 on each
+ * side); in and out must not overlap. */
+int lp2d_expand(int n, int n_out, int batch, int dtype, const void* in, void* out,
+                void* stream);
+/* Beam2D.coarse (beam2d.py:437-460): block mean over mean_order x mean_order,
+ * (batch, n, n) -> (batch, n/mean_order, n/mean_order); mean_order must divide n. */
+int lp2d_coarse(int n, int mean_order, int batch, int dtype, const void* in, void* out,
+                void* stream);
+
 /* ---------------------------------------------------------------- field -> field */
 /* field_out = ifft2(H(z) * aperture * fft2(field_in * exp(i*phase) * lens)), fused;
  * spectrum_out (may be NULL) receives H*aperture*fft2(...), i.e. the reference's

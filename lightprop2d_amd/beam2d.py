@@ -178,6 +178,28 @@ class Beam2D:
         self.field = torch.empty_like(self.spectrum)
         P.spectral_apply(self.spectrum, masks, field_out=self.field)
 
+    def expand(self, area_size: float):
+        """beam2d.py:375-404: enlarge the area at constant dL, replicating the edge rows
+        and columns of the field; spectrum = fft2(field)."""
+        old_npoints = self.npoints
+        npoints = int(self.npoints * area_size / self.area_size)
+        npoints -= npoints % 2
+        field = P.expand(self.field, npoints)
+        self.npoints = npoints
+        self.area_size = area_size
+        self._construct_grids()
+        self.field = field
+        self.spectrum = self._fft2(self.field)
+
+    def coarse(self, mean_order: int = 1):
+        """beam2d.py:437-460: block-average the field over mean_order x mean_order
+        (npoints //= mean_order); spectrum = fft2(field)."""
+        field = P.coarse(self.field, mean_order)
+        self.npoints //= mean_order
+        self._construct_grids()
+        self.field = field
+        self.spectrum = self._fft2(self.field)
+
     def propagate(self, z: float):
         """beam2d.py:462-490: spectrum *= exp(-2*pi*i*frac(Kz*z)); field = ifft2(spectrum).
 

@@ -1,0 +1,106 @@
+// Grid resampling of Beam2D.expand / Beam2D.coarse (lightprop2d/beam2d.py:375-404,
+// 437-460) behind lp2d_expand / lp2d_coarse.  Both are one pass over the fields,
+// bound by HBM: expand reads n^2 and writes n_out^2 elements, coarse reads n^2 and
+// writes (n/m)^2.
+#include <hip/hip_runtime.h>
+
+#include "../../include/lp2d.h"
+
+extern "C" int lp2d_init(int device);
+namespace lp2d {
+int set_error(int code, const char* msg);
+}
+
+namespace {
+
+// Grid: x = output rows of all fields (batch * rows), y = 256-column chunks.  No
+// 64-bit division per element: each workgroup row knows its field and row.
+template <typename C>
+__global__ void k_expand(const C* __restrict__ in, C* __restrict__ out, int n, int n_out,
+                         int d) {
+  // edge replication: the d columns left/right repeat the first/last column, then
+  // the d rows above/below repeat the first/last (already widened) row
+  // (beam2d.py:393-401) -- element (j, i) of the result is in[clamp(j-d), clamp(i-d)]
+  const int row = blockIdx.x;
+  const int b = row / n_out, j = row - b * n_out;
+  const int i = blockIdx.y * blockDim.x + threadIdx.x;
+  if (i >= n_out) return;
+  int js = j - d, is = i - d;
+  js = js < 0 ? 0 : (js >= n ? n - 1 : js);
+  is = is < 0 ? 0 : (is >= n ? n - 1 : is);
+  out[(long long)row * n_out + i] = in[((long long)b * n + js) * n + is];
+}
+
+template <typename T, typename C>
+__global__ void k_coarse(const C* __restrict__ in, C* __restrict__ out, int n, int m) {
+  // block mean over m x m (beam2d.py:456-459: reshape (N, m, N, m).mean((1, 3))),
+  // accumulated in the field's precision, row of the block by row
+  const int no = n / m;
+  const int row = blockIdx.x;
+  const int b = row / no, J = row - b * no;
+  const int I = blockIdx.y * blockDim.x + threadIdx.x;
+  if (I >= no) return;
+  const C* src = in + ((long long)b * n + (long long)J * m) * n + (long long)I * m;
+  T sr = 0, si = 0;
+  for (int u = 0; u < m; ++u)
+    for (int v = 0; v < m; ++v) {
+      const C c = src[(long long)u * n + v];
+      sr += c.x;
+      si += c.y;
+    }
+  const T cnt = (T)(m * m);
+  C o;
+  o.x = sr / cnt;  // numpy's mean: sum / count
+  o.y = si / cnt;
+  out[(long long)row * no + I] = o;
+}
+
+int prep() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return lp2d::set_error(LP2D_E_HIP, "hipGetDevice failed");
+  return lp2d_init(dev);
+}
+
+
+}  // namespace
+
+extern "C" int lp2d_expand(int n, int n_out, int batch, int dtype, const void* in, void* out,
+                           void* stream) {
+  using lp2d::set_error;
+  if (n < 1 || n_out < n || batch < 1 || ((n_out - n) & 1))
+    return set_error(LP2D_E_SIZE, "expand needs 1 <= n <= n_out with n_out - n even");
+  if (dtype != LP2D_C64 && dtype != LP2D_C128) return set_error(LP2D_E_DTYPE, "unknown dtype");
+  if (!in || !out) return set_error(LP2D_E_ARG, "null pointer");
+  if (int rc = prep()) return rc;
+  if ((long long)batch * n_out > 0x7fffffffLL) return set_error(LP2D_E_SIZE, "batch too large");
+  const int d = (n_out - n) / 2;
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 grid((unsigned)(batch * n_out), (unsigned)((n_out + 255) / 256));
+  if (dtype == LP2D_C64)
+    k_expand<float2><<<grid, 256, 0, s>>>((const float2*)in, (float2*)out, n, n_out, d);
+  else
+    k_expand<double2><<<grid, 256, 0, s>>>((const double2*)in, (double2*)out, n, n_out, d);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? LP2D_OK : set_error(LP2D_E_HIP, hipGetErrorString(e));
+}
+
+extern "C" int lp2d_coarse(int n, int mean_order, int batch, int dtype, const void* in,
+                           void* out, void* stream) {
+  using lp2d::set_error;
+  if (n < 1 || mean_order < 1 || batch < 1 || n % mean_order)
+    return set_error(LP2D_E_SIZE, "coarse needs mean_order >= 1 dividing n");
+  if (dtype != LP2D_C64 && dtype != LP2D_C128) return set_error(LP2D_E_DTYPE, "unknown dtype");
+  if (!in || !out) return set_error(LP2D_E_ARG, "null pointer");
+  if (int rc = prep()) return rc;
+  const int no = n / mean_order;
+  if ((long long)batch * no > 0x7fffffffLL) return set_error(LP2D_E_SIZE, "batch too large");
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 grid((unsigned)(batch * no), (unsigned)((no + 255) / 256));
+  if (dtype == LP2D_C64)
+    k_coarse<float, float2><<<grid, 256, 0, s>>>((const float2*)in, (float2*)out, n, mean_order);
+  else
+    k_coarse<double, double2><<<grid, 256, 0, s>>>((const double2*)in, (double2*)out, n,
+                                                   mean_order);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? LP2D_OK : set_error(LP2D_E_HIP, hipGetErrorString(e));
+}

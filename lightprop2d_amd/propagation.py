@@ -17,7 +17,7 @@ from ._lib import Lp2dError
 
 __all__ = ["fft2", "ifft2", "kz_grid", "spatial_apply", "spectral_apply", "propagate",
            "propagate_adjoint", "intensity", "sum_planes", "UnevenBatch", "optics",
-           "Lp2dError"]
+           "expand", "coarse", "Lp2dError"]
 
 _DT = {torch.complex64: L.LP2D_C64, torch.complex128: L.LP2D_C128}
 _MASK_DT = {torch.bool: L.LP2D_MASK_U8, torch.uint8: L.LP2D_MASK_U8,
@@ -277,6 +277,34 @@ def intensity(field: torch.Tensor, out: Optional[torch.Tensor] = None):
     with torch.cuda.device(field.device):
         L.check(lib.lp2d_intensity(field.numel(), _DT[field.dtype], _ptr(field), _ptr(out),
                                    _stream(field)), "intensity")
+    return out
+
+
+def expand(field: torch.Tensor, n_out: int) -> torch.Tensor:
+    """Edge-replicating enlargement (n, n) -> (n_out, n_out), n_out - n even
+    (Beam2D.expand, beam2d.py:389-401); batched over leading fields."""
+    lib = _lib()
+    n, batch, dt = _field_geom(field)
+    n_out = int(n_out)
+    out = torch.empty(field.shape[:-2] + (n_out, n_out), dtype=field.dtype, device=field.device)
+    with torch.cuda.device(field.device):
+        L.check(lib.lp2d_expand(n, n_out, batch, dt, _ptr(field), _ptr(out), _stream(field)),
+                "expand")
+    return out
+
+
+def coarse(field: torch.Tensor, mean_order: int) -> torch.Tensor:
+    """Block mean over mean_order x mean_order (Beam2D.coarse, beam2d.py:456-459)."""
+    lib = _lib()
+    n, batch, dt = _field_geom(field)
+    m = int(mean_order)
+    if m < 1 or n % m:
+        raise ValueError(f"cannot reshape a {n} x {n} field into blocks of {m} x {m}")
+    out = torch.empty(field.shape[:-2] + (n // m, n // m), dtype=field.dtype,
+                      device=field.device)
+    with torch.cuda.device(field.device):
+        L.check(lib.lp2d_coarse(n, m, batch, dt, _ptr(field), _ptr(out), _stream(field)),
+                "coarse")
     return out
 
 

@@ -308,6 +308,19 @@ def propagate_field(u, area_size, wl, z, phase_mask=None, aperture_radius=None,
     return ifft2(S)
 
 
+def propagate_field_threaded(u, area_size, wl, z, workers):
+    """``propagate_field`` (no mask, no lens) with scipy.fft on `workers` threads: the
+    stand-in for the reference's pyfftw branch with ``threads=-1`` (beam2d.py:27-29,
+    213-216), which is not installed.  Same grids and transfer function."""
+    import scipy.fft as sfft
+    u = np.asarray(u, dtype=np.complex128)
+    g = _grids(area_size, u.shape[-1], wl)
+    S = sfft.fft2(u, workers=workers)
+    if z != 0:
+        S = S * transfer_function(g.Kz, z)
+    return sfft.ifft2(S, workers=workers)
+
+
 def propagate_field_adjoint(grad_out, u_in, area_size, wl, z, phase_mask=None,
                             aperture_radius=None):
     """Adjoint of ``propagate_field`` for a real loss (PyTorch complex-gradient

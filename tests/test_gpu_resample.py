@@ -1,0 +1,70 @@
+"""expand / coarse (Beam2D.expand, Beam2D.coarse; beam2d.py:375-404, 437-460) on the
+device against the oracle.  The resampled fields are copies and means of the input:
+expand is bit-exact; coarse agrees to the rounding of an m*m-term mean (relative 1e-7
+in complex64, 1e-15 in complex128); spectra to the FFT bar (1e-5 / 1e-10)."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel_l2
+from oracle.angular_spectrum import OracleBeam2D
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def cplx(rs, shape, dt):
+    return (rs.standard_normal(shape) + 1j * rs.standard_normal(shape)).astype(dt)
+
+
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128], ids=["c64", "c128"])
+@pytest.mark.parametrize("n,n_out,batch", [(64, 128, 3), (256, 512, 2), (1024, 2048, 1),
+                                           (16, 30, 2), (8, 8, 1)])
+def test_expand_matches_edge_padding(dt, n, n_out, batch):
+    from lightprop2d_amd import propagation as P
+    u = cplx(np.random.RandomState(n), (batch, n, n), dt)
+    got = P.expand(torch.from_numpy(u).to(DEV), n_out).cpu().numpy()
+    d = (n_out - n) // 2
+    exp = np.stack([np.pad(u[i], d, mode="edge") for i in range(batch)])
+    np.testing.assert_array_equal(got, exp)
+
+
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128], ids=["c64", "c128"])
+@pytest.mark.parametrize("n,m,batch", [(64, 2, 3), (256, 4, 2), (2048, 2, 1), (48, 3, 2),
+                                       (16, 16, 1), (32, 1, 1)])
+def test_coarse_matches_block_mean(dt, n, m, batch):
+    from lightprop2d_amd import propagation as P
+    u = cplx(np.random.RandomState(n + m), (batch, n, n), dt)
+    got = P.coarse(torch.from_numpy(u).to(DEV), m).cpu().numpy()
+    exp = u.astype(np.complex128).reshape((batch, n // m, m, n // m, m)).mean((2, 4))
+    tol = 1e-6 if dt == np.complex64 else 1e-14
+    assert rel_l2(got, exp) < tol
+
+
+def test_coarse_rejects_a_non_divisor():
+    from lightprop2d_amd import propagation as P
+    with pytest.raises(ValueError):
+        P.coarse(torch.zeros((64, 64), dtype=torch.complex64, device=DEV), 3)
+
+
+@pytest.mark.parametrize("bits", [64, 128])
+def test_beam2d_expand_then_coarse_vs_oracle(bits):
+    from lightprop2d_amd import Beam2D, um, nm
+    from lightprop2d_amd import profiles as PP
+    n, area, wl = 256, 200 * um, 632 * nm
+    b = Beam2D(area, n, wl, init_field_gen=PP.gaussian_beam, init_gen_args=(1, 25 * um),
+               complex_bits=bits)
+    x = np.arange(-n // 2, n // 2, 1) * (area / n)
+    o = OracleBeam2D(area, n, wl, init_field=np.exp(-(x[None, :] ** 2 + x[:, None] ** 2)
+                                                    / (25 * um) ** 2))
+    tol = 1e-5 if bits == 64 else 1e-10
+    for op, arg in (("propagate", 300 * um), ("expand", 2 * area), ("propagate", 500 * um),
+                    ("coarse", 2), ("propagate", 200 * um)):
+        getattr(b, op)(arg)
+        getattr(o, op)(arg)
+        assert b.npoints == o.npoints and abs(b.area_size - o.area_size) < 1e-15
+        assert abs(b.dL - o.dL) < 1e-18
+        assert rel_l2(b.field.cpu().numpy(), o.field) < tol, op
+        assert rel_l2(b.spectrum.cpu().numpy(), o.spectrum) < tol, op
+    assert tuple(b.X.shape) == (n,) and tuple(b.Kz.shape) == (n, n)

@@ -286,3 +286,53 @@ def test_c64_state_close_to_c128():
         b.propagate(50 * um)
     assert a.field.dtype == np.complex64
     assert rel_l2(a.field, b.field) < 1e-5
+
+
+def test_expand_is_edge_padding():
+    # beam2d.py:385-404 restated: equals numpy.pad(mode="edge"), dL kept, spectrum = fft2
+    rs = np.random.RandomState(4)
+    n = 64
+    b = OracleBeam2D(100e-4, n, 632e-7,
+                     init_field=rs.standard_normal((n, n)) + 1j * rs.standard_normal((n, n)))
+    f0 = b.field.copy()
+    b.expand(200e-4)
+    assert b.npoints == 128 and b.area_size == 200e-4
+    assert abs(b.dL - 100e-4 / 64) < 1e-18
+    np.testing.assert_array_equal(b.field, np.pad(f0, 32, mode="edge"))
+    np.testing.assert_allclose(b.spectrum, np.fft.fft2(b.field), rtol=0, atol=1e-12)
+    # odd growth is rounded down to an even npoints (beam2d.py:387)
+    c = OracleBeam2D(100e-4, 64, 632e-7, init_field=f0)
+    c.expand(151e-4)
+    assert c.npoints == 96 and c.field.shape == (96, 96)
+    np.testing.assert_array_equal(c.field, np.pad(f0, 16, mode="edge"))
+
+
+def test_coarse_is_block_mean():
+    # beam2d.py:454-460 restated: equals an explicit loop over m x m blocks
+    rs = np.random.RandomState(5)
+    n, m = 48, 3
+    f0 = rs.standard_normal((n, n)) + 1j * rs.standard_normal((n, n))
+    b = OracleBeam2D(100e-4, n, 632e-7, init_field=f0)
+    b.coarse(m)
+    assert b.npoints == n // m
+    exp = np.empty((n // m, n // m), complex)
+    for J in range(n // m):
+        for I in range(n // m):
+            exp[J, I] = sum(f0[J * m + u, I * m + v] for u in range(m) for v in range(m)) / m ** 2
+    np.testing.assert_allclose(b.field, exp, rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(b.spectrum, np.fft.fft2(b.field), rtol=0, atol=1e-12)
+    # a constant field stays constant through expand + coarse
+    c = OracleBeam2D(100e-4, 32, 632e-7, init_field=np.full((32, 32), 2 - 1j))
+    c.expand(200e-4)
+    c.coarse(2)
+    np.testing.assert_allclose(c.field, np.full((32, 32), 2 - 1j), rtol=0, atol=1e-15)
+
+
+def test_threaded_baseline_matches_the_oracle():
+    # the multi-threaded CPU baseline computes the same propagation
+    from oracle.angular_spectrum import propagate_field_threaded
+    rs = np.random.RandomState(8)
+    u = rs.standard_normal((128, 128)) + 1j * rs.standard_normal((128, 128))
+    a = oracle.propagate_field(u, 200e-4, 632e-7, 3e-3)
+    b = propagate_field_threaded(u, 200e-4, 632e-7, 3e-3, 2)
+    assert rel_l2(b, a) < 1e-13
