@@ -180,7 +180,42 @@ def extra_legs(torch, P, L, lib, dev):
         "ms_per_chain": round(el * 1e3, 4)}
     out["generators"] = generator_leg(torch, dev)
     out["resample"] = resample_leg(torch, P, dev)
+    out["modes"] = modes_leg(torch, P, dev)
     return out
+
+
+def modes_leg(torch, P, dev):
+    """Mode-basis projection and reconstruction: 64 modes of 256^2 points against 32
+    complex64 fields of 512^2 (central 256^2 window).  Algorithmic bytes: the mode
+    matrix once plus the field windows (projection) or the fields (reconstruction)."""
+    M, nb, n, B = 64, 256, 512, 32
+    g = torch.Generator(device="cpu").manual_seed(4)
+    modes = torch.randn((M, nb * nb), dtype=torch.complex64, generator=g).to(dev)
+    f = torch.randn((B, n, n), dtype=torch.complex64, generator=g).to(dev)
+    c = torch.randn((B, M), dtype=torch.complex128, generator=g).to(dev)
+    res = {}
+    for name, call, alg in (("project", lambda: P.modes_project(f, modes),
+                             8 * M * nb * nb + 8 * B * nb * nb + 16 * B * M),
+                            ("construct", lambda: P.modes_construct(modes, c, n, torch.complex64,
+                                                                   dev),
+                             8 * M * nb * nb + 8 * B * n * n + 16 * B * M)):
+        call()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(20):
+            call()
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / 20
+        res[name] = {"us_per_call": round(el * 1e6, 1), "GBs": round(alg / el / 1e9, 1),
+                     "hbm_frac": round(alg / el / 1e9 / HBM_PEAK_GBS, 4)}
+    mh = modes[:, :].cpu().numpy().astype(np.complex128)
+    fh = f[0, 128:384, 128:384].reshape(-1).cpu().numpy().astype(np.complex128)
+    t0 = time.perf_counter()
+    mh @ fh
+    res["numpy_one_field_project_us"] = round((time.perf_counter() - t0) * 1e6, 1)
+    res["workload"] = ("64 modes x 256^2, 32 complex64 fields of 512^2 (central window); "
+                       "coefficients complex128")
+    return res
 
 
 def resample_leg(torch, P, dev):

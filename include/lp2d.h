@@ -167,6 +167,20 @@ int lp2d_expand(int n, int n_out, int batch, int dtype, const void* in, void* ou
 int lp2d_coarse(int n, int mean_order, int batch, int dtype, const void* in, void* out,
                 void* stream);
 
+/* ---------------------------------------------------------------- mode bases */
+/* Mode-basis projection (Beam2D.deconstruct_by_modes 'scalar', beam2d.py:563-603):
+ * coeffs[b, k] = sum_p modes[k, p] * field[b, window(p)] -- the reference's
+ * unconjugated product of the (nmodes, nb*nb) mode matrix with the central nb x nb
+ * window (offset (n - nb)/2) of each n x n field, flattened row-major.  modes and field
+ * in `dtype`; coeffs is (batch, nmodes) complex128 (FP64 accumulation), overwritten. */
+int lp2d_modes_project(int n, int nb, int nmodes, int batch, int dtype, const void* modes,
+                       const void* field, void* coeffs, void* stream);
+/* Reconstruction (Beam2D.construct_by_modes, beam2d.py:653-667): field[b] =
+ * sum_k coeffs[b, k] * mode_k zero-padded into the centre of the n x n grid.  coeffs
+ * (batch, nmodes) complex128; field (batch, n, n) in `dtype`, overwritten. */
+int lp2d_modes_construct(int n, int nb, int nmodes, int batch, int dtype, const void* modes,
+                         const void* coeffs, void* field, void* stream);
+
 /* ---------------------------------------------------------------- field -> field */
 /* field_out = ifft2(H(z) * aperture * fft2(field_in * exp(i*phase) * lens)), fused;
  * spectrum_out (may be NULL) receives H*aperture*fft2(...), i.e. the reference's

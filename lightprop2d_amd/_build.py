@@ -21,7 +21,7 @@ LIB = os.path.join(PKG, "liblp2d.so")
 
 SOURCES = ["lp2d_f32_b.hip", "lp2d_f32_c.hip", "lp2d_f64_b.hip", "lp2d_f64_c.hip",
            "lp2d_f32_a.hip", "lp2d_f64_a.hip", "lp2d_api.hip", "lp2d_profiles.hip",
-           "lp2d_resample.hip"]
+           "lp2d_resample.hip", "lp2d_modes.hip"]
 
 ARCH = os.environ.get("LP2D_ARCH", "gfx950")
 # Packed FP32 instructions are disabled for all device code (the compiler would

@@ -40,7 +40,8 @@ EXPORTED = [
     "lp2d_batch_run", "lp2d_batch_destroy", "lp2d_variant_count",
     "lp2d_set_variant", "lp2d_get_variant", "lp2d_variant_name", "lp2d_trace_enable",
     "lp2d_trace_collect", "lp2d_moments", "lp2d_set_chunk", "lp2d_get_chunk",
-    "lp2d_profile", "lp2d_expand", "lp2d_coarse",
+    "lp2d_profile", "lp2d_expand", "lp2d_coarse", "lp2d_modes_project",
+    "lp2d_modes_construct",
 ]
 
 # generator kinds of lp2d_profile
@@ -124,6 +125,8 @@ _SIGS = {
     "lp2d_profile": (_I, [_I, _I, _I, _P, _P, C.POINTER(lp2d_profile_params), _P, _P, _I, _P]),
     "lp2d_expand": (_I, [_I, _I, _I, _I, _P, _P, _P]),
     "lp2d_coarse": (_I, [_I, _I, _I, _I, _P, _P, _P]),
+    "lp2d_modes_project": (_I, [_I, _I, _I, _I, _I, _P, _P, _P, _P]),
+    "lp2d_modes_construct": (_I, [_I, _I, _I, _I, _I, _P, _P, _P, _P]),
 }
 
 

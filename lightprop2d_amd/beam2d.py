@@ -200,6 +200,41 @@ class Beam2D:
         self.field = field
         self.spectrum = self._fft2(self.field)
 
+    # ------------------------------------------------------------- mode bases
+    def _expand_basis(self, modes_list) -> torch.Tensor:
+        """beam2d.py:538-561: flattened Nb x Nb modes zero-padded into the centre of
+        the npoints grid, (nmodes, npoints, npoints)."""
+        eye = torch.eye(len(modes_list), dtype=torch.complex128, device=self._device)
+        return P.modes_construct(modes_list, eye, self.npoints, self.complex, self._device)
+
+    def deconstruct_by_modes(self, modes_list, mode: str = "scalar") -> torch.Tensor:
+        """beam2d.py:563-603: 'scalar' = the unconjugated product of the mode matrix with
+        the central window of the field (HIP kernel); 'lstsq' = least squares of
+        modes^T c = field (torch.linalg on the device)."""
+        if mode == "scalar":
+            self.modes_coeffs = P.modes_project(self.field, modes_list)
+        elif mode == "lstsq":
+            m = P.mode_matrix(modes_list, torch.complex128, self._device)
+            nb = int(round(m.shape[1] ** 0.5))
+            d = (self.npoints - nb) // 2
+            f = self.field[d:self.npoints - d, d:self.npoints - d].reshape(-1, 1)
+            self.modes_coeffs = torch.linalg.lstsq(m.T, f.to(torch.complex128)).solution[:, 0]
+        return self.modes_coeffs
+
+    def fast_deconstruct_by_modes(self, modes_matrix_t, modes_matrix_dot_t) -> torch.Tensor:
+        """beam2d.py:605-651: lstsq(Gram matrix, modes_matrix_t . field), the product by
+        the projection kernel, the small solve with torch.linalg on the device."""
+        b = P.modes_project(self.field, modes_matrix_t)
+        g = self._asxp(modes_matrix_dot_t).to(torch.complex128)
+        self.modes_coeffs = torch.linalg.lstsq(g, b.reshape(-1, 1)).solution[:, 0]
+        return self.modes_coeffs
+
+    def construct_by_modes(self, modes_list, modes_coeffs):
+        """beam2d.py:653-667: field = sum_i c_i M_i (zero-padded); spectrum = fft2."""
+        self.field = P.modes_construct(modes_list, modes_coeffs, self.npoints, self.complex,
+                                       self._device)
+        self.spectrum = self._fft2(self.field)
+
     def propagate(self, z: float):
         """beam2d.py:462-490: spectrum *= exp(-2*pi*i*frac(Kz*z)); field = ifft2(spectrum).
 

@@ -17,7 +17,8 @@ from ._lib import Lp2dError
 
 __all__ = ["fft2", "ifft2", "kz_grid", "spatial_apply", "spectral_apply", "propagate",
            "propagate_adjoint", "intensity", "sum_planes", "UnevenBatch", "optics",
-           "expand", "coarse", "Lp2dError"]
+           "expand", "coarse", "mode_matrix", "modes_project", "modes_construct",
+           "Lp2dError"]
 
 _DT = {torch.complex64: L.LP2D_C64, torch.complex128: L.LP2D_C128}
 _MASK_DT = {torch.bool: L.LP2D_MASK_U8, torch.uint8: L.LP2D_MASK_U8,
@@ -306,6 +307,63 @@ def coarse(field: torch.Tensor, mean_order: int) -> torch.Tensor:
         L.check(lib.lp2d_coarse(n, m, batch, dt, _ptr(field), _ptr(out), _stream(field)),
                 "coarse")
     return out
+
+
+def mode_matrix(modes, dtype, device) -> torch.Tensor:
+    """(nmodes, nb*nb) contiguous device tensor of `dtype` from a list of flattened
+    modes (pyMMF layout, beam2d.py:538-561), a 2-D array or a tensor."""
+    if isinstance(modes, torch.Tensor):
+        m = modes
+    elif isinstance(modes, np.ndarray):
+        m = torch.from_numpy(np.ascontiguousarray(modes))
+    else:
+        m = torch.from_numpy(np.ascontiguousarray(np.vstack([np.asarray(
+            x.detach().cpu().numpy() if isinstance(x, torch.Tensor) else x).reshape(1, -1)
+            for x in modes])))
+    if m.dim() != 2:
+        m = m.reshape(m.shape[0], -1)
+    nb = int(round(m.shape[1] ** 0.5))
+    if nb * nb != m.shape[1]:
+        raise Lp2dError(f"modes of {m.shape[1]} points are not square")
+    return m.to(device=device, dtype=dtype).contiguous()
+
+
+def modes_project(field: torch.Tensor, modes: torch.Tensor) -> torch.Tensor:
+    """coeffs[b, k] = sum_p modes[k, p] * field[b, window(p)] (unconjugated, as the
+    reference's ``modes_matrix @ flatten_field``, beam2d.py:597); complex128 result,
+    shape (nmodes,) for one field or (batch, nmodes)."""
+    lib = _lib()
+    n, batch, dt = _field_geom(field)
+    modes = mode_matrix(modes, field.dtype, field.device)
+    nm = modes.shape[0]
+    nb = int(round(modes.shape[1] ** 0.5))
+    out = torch.empty((batch, nm), dtype=torch.complex128, device=field.device)
+    with torch.cuda.device(field.device):
+        L.check(lib.lp2d_modes_project(n, nb, nm, batch, dt, _ptr(modes), _ptr(field), _ptr(out),
+                                       _stream(field)), "modes_project")
+    return out[0] if field.dim() == 2 else out
+
+
+def modes_construct(modes: torch.Tensor, coeffs, n: int, dtype=torch.complex64,
+                    device=None) -> torch.Tensor:
+    """field = sum_k coeffs[k] * mode_k zero-padded into an n x n grid
+    (beam2d.py:653-667); coeffs (nmodes,) or (batch, nmodes)."""
+    lib = _lib()
+    dev = device if device is not None else (modes.device if isinstance(modes, torch.Tensor)
+                                            else torch.device("cuda", torch.cuda.current_device()))
+    modes = mode_matrix(modes, dtype, dev)
+    nm = modes.shape[0]
+    nb = int(round(modes.shape[1] ** 0.5))
+    c = coeffs if isinstance(coeffs, torch.Tensor) else torch.from_numpy(np.asarray(coeffs))
+    c = c.to(device=dev, dtype=torch.complex128).contiguous()
+    single = c.dim() == 1
+    c = c.reshape(-1, nm)
+    batch = c.shape[0]
+    out = torch.empty((batch, n, n), dtype=dtype, device=dev)
+    with torch.cuda.device(dev):
+        L.check(lib.lp2d_modes_construct(n, nb, nm, batch, _DT[dtype], _ptr(modes), _ptr(c),
+                                         _ptr(out), _stream(out)), "modes_construct")
+    return out[0] if single else out
 
 
 class UnevenBatch:

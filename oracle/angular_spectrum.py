@@ -232,6 +232,51 @@ class OracleBeam2D:
             (self.npoints, mean_order, self.npoints, mean_order)).mean((1, 3))
         self.spectrum = fft2(self.field)
 
+    # ---------------------------------------------------------- mode bases
+    def _expand_basis(self, modes_list):
+        # beam2d.py:538-561: flattened Nb x Nb modes zero-padded into the centre of the
+        # npoints grid (or just reshaped when Nb == npoints)
+        Nb = int(np.sqrt(len(modes_list[0])))
+        if Nb != self.npoints:
+            dN = (self.npoints - Nb) // 2
+            Nt = self.npoints - dN
+            out = []
+            for m in modes_list:
+                me = np.zeros([self.npoints] * 2, dtype=np.complex128)
+                me[dN:Nt, dN:Nt] = np.asarray(m).reshape((Nb, Nb))
+                out.append(me)
+            return np.array(out)
+        return np.array([np.asarray(m).reshape((Nb, Nb)) for m in modes_list])
+
+    def _central_field(self, Nb):
+        dN = (self.npoints - Nb) // 2
+        return self.field[dN:self.npoints - dN, dN:self.npoints - dN].flatten()
+
+    def deconstruct_by_modes(self, modes_list, mode="scalar"):
+        # beam2d.py:563-603: coefficients by the (unconjugated) product with the mode
+        # matrix, or by least squares of modes^T c = field over the central Nb x Nb
+        modes_matrix = np.vstack([np.asarray(m) for m in modes_list])
+        Nb = int(np.sqrt(modes_matrix.shape[1]))
+        f = self._central_field(Nb)
+        if mode == "scalar":
+            self.modes_coeffs = modes_matrix @ f
+        elif mode == "lstsq":
+            self.modes_coeffs = np.linalg.lstsq(modes_matrix.T, f, rcond=-1)[0]
+        return self.modes_coeffs
+
+    def fast_deconstruct_by_modes(self, modes_matrix_t, modes_matrix_dot_t):
+        # beam2d.py:605-651: lstsq(Gram matrix, modes_matrix_t . field)
+        Nb = int(np.sqrt(modes_matrix_t.shape[1]))
+        f = self._central_field(Nb)
+        self.modes_coeffs = np.linalg.lstsq(modes_matrix_dot_t, modes_matrix_t.dot(f),
+                                            rcond=None)[0]
+        return self.modes_coeffs
+
+    def construct_by_modes(self, modes_list, modes_coeffs):
+        # beam2d.py:653-667: field = sum_i c_i M_i (padded), spectrum = fft2(field)
+        self.field = np.einsum('ijk,i->jk', self._expand_basis(modes_list), modes_coeffs)
+        self.spectrum = fft2(self.field)
+
     @property
     def iprofile(self):
         # beam2d.py:717

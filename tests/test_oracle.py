@@ -336,3 +336,37 @@ def test_threaded_baseline_matches_the_oracle():
     a = oracle.propagate_field(u, 200e-4, 632e-7, 3e-3)
     b = propagate_field_threaded(u, 200e-4, 632e-7, 3e-3, 2)
     assert rel_l2(b, a) < 1e-13
+
+
+def _orthonormal_modes(m, nb, seed, real=False):
+    # columns of a QR factor: an orthonormal set of m flattened nb x nb modes
+    rs = np.random.RandomState(seed)
+    a = rs.standard_normal((nb * nb, m))
+    if not real:
+        a = a + 1j * rs.standard_normal((nb * nb, m))
+    q, _ = np.linalg.qr(a)
+    return q.T.copy()  # (m, nb^2), rows orthonormal
+
+
+def test_modes_round_trip_and_padding():
+    # construct then deconstruct recovers the coefficients for a real orthonormal basis
+    # (the reference's unconjugated 'scalar' product is exact there), for any basis by
+    # least squares, and the padded basis is numpy.pad with zeros
+    n, nb, m = 48, 32, 12
+    modes = _orthonormal_modes(m, nb, 1, real=True)
+    c = np.random.RandomState(2).standard_normal(m) + 1j * np.random.RandomState(3).standard_normal(m)
+    b = OracleBeam2D(100e-4, n, 632e-7, init_field=np.zeros((n, n)))
+    b.construct_by_modes(list(modes), c)
+    e = b._expand_basis(list(modes))
+    np.testing.assert_array_equal(e[3], np.pad(modes[3].reshape(nb, nb), 8))
+    np.testing.assert_allclose(b.spectrum, np.fft.fft2(b.field), atol=1e-12)
+    np.testing.assert_allclose(b.deconstruct_by_modes(list(modes)), c, atol=1e-12)
+    cm = _orthonormal_modes(m, nb, 4) * np.exp(1j * 0.3)  # complex, not Hermitian-orthonormal
+    cm = cm + 0.2 * np.roll(cm, 1, axis=0)                # and not orthogonal at all
+    b.construct_by_modes(list(cm), c)
+    np.testing.assert_allclose(b.deconstruct_by_modes(list(cm), mode="lstsq"), c, atol=1e-10)
+    gram = cm.dot(cm.T)
+    np.testing.assert_allclose(b.fast_deconstruct_by_modes(cm, gram), c, atol=1e-9)
+    # a unit coefficient reconstructs its own padded mode
+    b.construct_by_modes(list(cm), np.eye(m)[5])
+    np.testing.assert_allclose(b.field, np.pad(cm[5].reshape(nb, nb), 8), atol=1e-15)
