@@ -194,8 +194,11 @@ def modes_leg(torch, P, dev):
     f = torch.randn((B, n, n), dtype=torch.complex64, generator=g).to(dev)
     c = torch.randn((B, M), dtype=torch.complex128, generator=g).to(dev)
     res = {}
+    f1 = f[0].contiguous()
     for name, call, alg in (("project", lambda: P.modes_project(f, modes),
                              8 * M * nb * nb + 8 * B * nb * nb + 16 * B * M),
+                            ("project_one_field", lambda: P.modes_project(f1, modes),
+                             8 * M * nb * nb + 8 * nb * nb + 16 * M),
                             ("construct", lambda: P.modes_construct(modes, c, n, torch.complex64,
                                                                    dev),
                              8 * M * nb * nb + 8 * B * n * n + 16 * B * M)):

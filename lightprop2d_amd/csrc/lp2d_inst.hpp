@@ -19,7 +19,7 @@ constexpr int kLdsBudget = 160 * 1024 - 2048;
 #define LP2D_OCC __attribute__((amdgpu_waves_per_eu(2)))
 template <typename T, int N, int E, int LINES, bool INV, int PS, bool FEAT>
 __global__ __launch_bounds__((N / E) * LINES) LP2D_OCC void k_rows(PassArgs<T> a) {
-  __shared__ cx<T> lds[LINES * PaddedLen<N, PS>::value];
+  __shared__ cx<T> lds[LINES * RowPitch<N, PS, LINES>::value];
   constexpr int TPF = (N + LINES - 1) / LINES;  // row tiles per field
   const int tile = blockIdx.x;
   const int b = tile / TPF;
@@ -27,17 +27,36 @@ __global__ __launch_bounds__((N / E) * LINES) LP2D_OCC void k_rows(PassArgs<T> a
   rows_body<T, N, E, LINES, INV, PS, FEAT>(a, b, y0, cmin(LINES, N - y0), lds, threadIdx.x);
 }
 
+// The pipelined gradient body when its screen factor is hoisted and the LDS twiddle
+// tables fit beside the row tiles.
 template <typename T, int N, int E, int LINES, int PS>
+struct GradPipe {
+  static constexpr bool value =
+      LINES * PaddedLen<N, PS>::value * (int)sizeof(cx<T>) +
+          TwlLen<N, E>::value * (int)sizeof(cx<T>) <= kLdsBudget;
+};
+
+template <typename T, int N, int E, int LINES, int PS, bool HOIST>
 __global__ __launch_bounds__((N / E) * LINES) LP2D_OCC void k_rows_grad(PassArgs<T> a, int groups,
                                                                 float* partial) {
+  constexpr bool PIPE = HOIST && GradPipe<T, N, E, LINES, PS>::value;
   __shared__ cx<T> lds[LINES * PaddedLen<N, PS>::value];
+  __shared__ cx<T> twl[PIPE ? TwlLen<N, E>::value : 1];
   constexpr int TPF = (N + LINES - 1) / LINES;
   const int tile = blockIdx.x % TPF, g = blockIdx.x / TPF;
   const int per = (a.batch + groups - 1) / groups;
   const int b0 = g * per, b1 = cmin(a.batch, b0 + per);
   const int y0 = tile * LINES;
-  rows_grad_body<T, N, E, LINES, PS>(a, b0, b1, y0, cmin(LINES, N - y0),
-                                     partial + (long long)g * N * N, lds, threadIdx.x);
+  if constexpr (PIPE) {
+    twl_fill<T, N, E, (N / E) * LINES>(twl, a.tw, threadIdx.x);
+    __syncthreads();
+    rows_grad_pipe_body<T, N, E, LINES, PS>(a, b0, b1, y0, cmin(LINES, N - y0),
+                                            partial + (long long)g * N * N, lds, twl,
+                                            threadIdx.x);
+  } else {
+    rows_grad_body<T, N, E, LINES, PS, HOIST>(a, b0, b1, y0, cmin(LINES, N - y0),
+                                              partial + (long long)g * N * N, lds, threadIdx.x);
+  }
 }
 
 // Grid: (N/W strips) x (field groups of `per` fields).
@@ -79,6 +98,22 @@ __global__ __launch_bounds__((N / E) * W) LP2D_OCC void k_cols_pipe(PassArgs<T> 
   cols_pipe_body<T, N, E, W, PS>(a, b0, b1, x0, lds, twl, pf, threadIdx.x);
 }
 
+// Whole field in LDS, pipelined over groups of `per` <= kPipeMaxPer fields.
+template <typename T, int N, int E, int PS>
+__global__ __launch_bounds__(N * (N / E)) void k_small_pipe(PassArgs<T> a, int per) {
+  __shared__ cx<T> lds[N * SmallPitch<N, PS>::value];
+  __shared__ cx<T> twl[TwlLen<N, E>::value];
+  __shared__ PipeField pf[kPipeMaxPer];
+  constexpr int THREADS = N * (N / E);
+  static_assert(THREADS >= kPipeMaxPer, "one thread per staged field");
+  const int b0 = blockIdx.x * per;
+  const int b1 = cmin(a.batch, b0 + per);
+  twl_fill<T, N, E, THREADS>(twl, a.tw, threadIdx.x);
+  pipe_stage_fields(a, b0, b1, pf, threadIdx.x);
+  __syncthreads();
+  small_pipe_body<T, N, E, PS>(a, b0, b1, lds, twl, pf, threadIdx.x);
+}
+
 template <typename T, int N, int E, int PS>
 __global__ __launch_bounds__(N * (N / E)) void k_small(PassArgs<T> a, int mode) {
   __shared__ cx<T> lds[N * SmallPitch<N, PS>::value];
@@ -92,7 +127,7 @@ struct RowShape {
   static constexpr int PS = cmax(1, ilog2c(E));
   static constexpr int LINES = cmax(1, cmin(N, 256 / TL));
   static constexpr int THREADS = TL * LINES;
-  static constexpr int LDS = LINES * PaddedLen<N, PS>::value * (int)sizeof(cx<T>);
+  static constexpr int LDS = LINES * RowPitch<N, PS, LINES>::value * (int)sizeof(cx<T>);
 };
 
 template <typename T, int N, int E, int WT>
@@ -138,7 +173,7 @@ hipError_t launch_rows(const PassArgs<T>& a, bool inv, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <typename T, int N, int E, int LINES, int PS>
+template <typename T, int N, int E, int LINES, int PS, bool HOIST>
 hipError_t launch_rows_grad(const PassArgs<T>& a, int groups, float* partial, hipStream_t s) {
   // reading a tiled plane in place needs whole 8-row blocks unless nothing is stored
   if ((a.flags & F_TILED) &&
@@ -146,7 +181,8 @@ hipError_t launch_rows_grad(const PassArgs<T>& a, int groups, float* partial, hi
     return hipErrorInvalidValue;
   constexpr int threads = (N / E) * LINES;
   constexpr int TPF = (N + LINES - 1) / LINES;
-  k_rows_grad<T, N, E, LINES, PS><<<(unsigned)(TPF * groups), threads, 0, s>>>(a, groups, partial);
+  k_rows_grad<T, N, E, LINES, PS, HOIST><<<(unsigned)(TPF * groups), threads, 0, s>>>(a, groups,
+                                                                                   partial);
   return hipGetLastError();
 }
 
@@ -240,8 +276,32 @@ hipError_t launch_cols_pipe(const PassArgs<T>& a, int mode, hipStream_t s) {
   return launch_cols<T, N, E, W, PS, false>(a, mode, s);
 }
 
+// The fused complex64 propagation (transfer function / pupil / scale / spectrum
+// output only) runs the pipelined whole-field kernel; every other combination the
+// general one.
 template <typename T, int N, int E, int PS>
 hipError_t launch_small(const PassArgs<T>& a, int mode, hipStream_t s) {
+  constexpr int PIPE_OK_FLAGS = F_H | F_APERTURE | F_SCALE | F_SPEC_OUT;
+  if constexpr (sizeof(T) == 4 && N >= 32 && E == 16) {
+    constexpr int threads = N * (N / E);
+    constexpr int lds = N * SmallPitch<N, PS>::value * (int)sizeof(cx<T>) +
+                        TwlLen<N, E>::value * (int)sizeof(cx<T>) +
+                        kPipeMaxPer * (int)sizeof(PipeField);
+    if constexpr (lds <= kLdsBudget) {
+      if (mode == (MODE_FWD | MODE_INV) && a.evan == EV_PASS &&
+          (a.flags & ~PIPE_OK_FLAGS) == 0) {
+        static int slots = 0;  // same value on every thread that races to set it
+        if (slots == 0) slots = resident_groups((const void*)&k_small_pipe<T, N, E, PS>, threads);
+        int groups = slots < a.batch ? slots : a.batch;
+        if (groups < 1) groups = 1;
+        int per = (a.batch + groups - 1) / groups;
+        if (per > kPipeMaxPer) per = kPipeMaxPer;
+        groups = (a.batch + per - 1) / per;
+        k_small_pipe<T, N, E, PS><<<(unsigned)groups, threads, 0, s>>>(a, per);
+        return hipGetLastError();
+      }
+    }
+  }
   k_small<T, N, E, PS><<<(unsigned)a.batch, N * (N / E), 0, s>>>(a, mode);
   return hipGetLastError();
 }
@@ -256,8 +316,8 @@ void add_row(Registry<T>& r) {
   // (taller gradient tiles, for reading a block-tiled plane in place, measured slower:
   // a 1024-thread workgroup per CU stalls the CU at every exchange barrier)
   r.rows[lg][k++] = RowVar<T>{&launch_rows<T, N, E, S::LINES, S::PS>,
-                              &launch_rows_grad<T, N, E, S::LINES, S::PS>, E, S::LINES,
-                              S::THREADS, S::LDS, S::LINES};
+                              &launch_rows_grad<T, N, E, S::LINES, S::PS, (E <= 8)>, E,
+                              S::LINES, S::THREADS, S::LDS, S::LINES};
 }
 
 template <typename T, int N, int E, int WT, bool HOIST = false>

@@ -430,12 +430,30 @@ __device__ __forceinline__ int valid_row(int y, int ylast) { return y <= ylast ?
 // covers contiguous memory.  lds: LINES * PaddedLen<N,PS> elements.
 // FEAT = false compiles the plain pass (load, FFT, optional scale, store): no
 // prologue/epilogue code, so its register budget is the FFT's alone.
+// LDS row pitch of a row tile: for tiles of 8 rows or more the padded line plus 4
+// elements, so that the 8 rows of one wave (the block-tiled read below) start in
+// different banks.
+template <int N, int PS, int LINES>
+struct RowPitch {
+  static constexpr int value = PaddedLen<N, PS>::value + (LINES >= 8 ? 4 : 0);
+};
+
 template <typename T, int N, int E, int LINES, bool INV, int PS, bool FEAT = true>
 __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, int nrows,
                                           cx<T>* lds, int tid) {
   constexpr int TL = N / E;
-  constexpr int L = PaddedLen<N, PS>::value;
-  const int l = tid / TL, t = tid - (tid / TL) * TL;
+  constexpr int L = RowPitch<N, PS, LINES>::value;
+  // Thread -> (row l, lane t).  Row-major planes: each row's TL threads consecutive.
+  // An inverse pass reading a block-tiled plane: each wave takes 8 rows x 8 lanes, so
+  // every load instruction of the wave reads one whole 8x8 tile (512 contiguous bytes).
+  int l = tid / TL, t = tid - (tid / TL) * TL;
+  if constexpr (INV && LINES % kTileRows == 0 && TL % kTileCols == 0) {
+    if (a.flags & F_TILED) {
+      const int g = tid / (kTileRows * TL), r = tid - g * (kTileRows * TL);
+      l = g * kTileRows + ((r & 63) >> 3);
+      t = (r >> 6) * kTileCols + (r & 7);
+    }
+  }
   const int y = valid_row(y0 + l, y0 + nrows - 1);
   const bool valid = l < nrows;
   const long long frow = (long long)b * a.fstride + (long long)y * N;
@@ -499,7 +517,10 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
 // summed over the fields in registers and written once to `partial` (n x n).  The
 // screen factor conj(exp(i*phase) * lens) is the same for every field, so it is
 // computed once per pixel before the field loop.
-template <typename T, int N, int E, int LINES, int PS>
+// HOIST: the screen factor of the thread's E elements is computed once and kept in
+// registers across the group's fields (small E); otherwise it is recomputed per field
+// (large E, where holding it would spill).
+template <typename T, int N, int E, int LINES, int PS, bool HOIST = true>
 __device__ __forceinline__ void rows_grad_body(const PassArgs<T>& a, int b0, int b1, int y0,
                                                int nrows, float* __restrict__ partial,
                                                cx<T>* lds, int tid) {
@@ -510,11 +531,11 @@ __device__ __forceinline__ void rows_grad_body(const PassArgs<T>& a, int b0, int
   const bool valid = l < nrows;
   const bool store = (a.flags & F_NO_STORE) == 0;
   LineAcc<T, PS> acc{lds + l * L};
-  cx<T> ec[E];
+  cx<T> ec[HOIST ? E : 1];
   float gsum[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
-    ec[m] = grad_fold(a, mk(a.scale, (T)0), 0, y, t + TL * m, N);
+    if constexpr (HOIST) ec[m] = grad_fold(a, mk(a.scale, (T)0), 0, y, t + TL * m, N);
     gsum[m] = 0.f;
   }
   // block-tiled offset of (y, x = t + TL*m) in the field: lo_t + 8*TL*m (TL a multiple
@@ -535,10 +556,14 @@ __device__ __forceinline__ void rows_grad_body(const PassArgs<T>& a, int b0, int
     }
     line_fft<T, N, E, true>(v, t, acc, a.tw_inv + (N - 2));
     const cx<T>* __restrict__ us = a.u_saved + (long long)b * a.u_sb + (long long)y * N;
+    int yy = y, tt = t;
+    if constexpr (!HOIST) asm volatile("" : "+v"(yy), "+v"(tt));  // keep it per field
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int x = t + TL * m;
-      const cx<T> g = cmul(v[m], ec[m]);
+      const cx<T> g =
+          HOIST ? cmul(v[m], ec[HOIST ? m : 0])
+                : cmul(v[m], grad_fold(a, mk(a.scale, (T)0), 0, yy, tt + TL * m, N));
       const cx<T> u = us[x];
       gsum[m] += (float)(u.re * g.im - u.im * g.re);
       if (store && valid) a.dst[frow + x] = g;
@@ -625,6 +650,72 @@ __device__ __forceinline__ void cols_body(const PassArgs<T>& a, int b0, int b1, 
     const T sc = (a.flags & F_SCALE) ? a.scale : (T)1;
 #pragma unroll
     for (int m = 0; m < E; ++m) dst[lo + (unsigned)(TL * N * m)] = mk(v[m].re * sc, v[m].im * sc);
+  }
+}
+
+// The gradient pass above, software-pipelined over the group's fields: the rows of
+// field b+1 (the plane and the saved forward input) are loaded into registers while
+// field b is transformed; twiddles come from LDS (twl) and the first field is peeled
+// off the loop, so every wait in the loop covers the prefetch only.  Screen factor
+// hoisted (small E).
+template <typename T, int N, int E, int LINES, int PS>
+__device__ __forceinline__ void rows_grad_pipe_body(const PassArgs<T>& a, int b0, int b1, int y0,
+                                                    int nrows, float* __restrict__ partial,
+                                                    cx<T>* lds, const cx<T>* twl, int tid) {
+  constexpr int TL = N / E;
+  constexpr int L = PaddedLen<N, PS>::value;
+  const int l = tid / TL, t = tid - (tid / TL) * TL;
+  const int y = valid_row(y0 + l, y0 + nrows - 1);
+  const bool valid = l < nrows;
+  const bool store = (a.flags & F_NO_STORE) == 0;
+  const bool tiled = (a.flags & F_TILED) != 0;
+  LineAcc<T, PS> acc{lds + l * L};
+  cx<T> ec[E];
+  float gsum[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    ec[m] = grad_fold(a, mk(a.scale, (T)0), 0, y, t + TL * m, N);
+    gsum[m] = 0.f;
+  }
+  const unsigned lo_n = (unsigned)(y * N + t);
+  const unsigned lo_t = (unsigned)((y >> 3) * kTileRows * N + (t >> 3) * (kTileRows * kTileCols) +
+                                   (y & 7) * kTileCols + (t & 7));
+  cx<T> cur[E], nxt[E], ucur[E], unxt[E];
+  auto fetch = [&](int bf) {
+    const LP2D_GLB cx<T>* __restrict__ src = glb(a.src + (long long)bf * a.fstride);
+    const LP2D_GLB cx<T>* __restrict__ us = glb(a.u_saved + (long long)bf * a.u_sb);
+    if (tiled) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) nxt[m] = src[lo_t + (unsigned)(kTileRows * TL * m)];
+    } else {
+#pragma unroll
+      for (int m = 0; m < E; ++m) nxt[m] = src[lo_n + (unsigned)(TL * m)];
+    }
+#pragma unroll
+    for (int m = 0; m < E; ++m) unxt[m] = us[lo_n + (unsigned)(TL * m)];
+  };
+  auto step = [&](int b) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      cur[m] = nxt[m];
+      ucur[m] = unxt[m];
+    }
+    if (b + 1 < b1) fetch(b + 1);  // uniform branch
+    line_fft_l<T, N, E, true>(cur, t, acc, twl);
+    LP2D_GLB cx<T>* __restrict__ dst = glb(a.dst + (long long)b * a.fstride);
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const cx<T> g = cmul(cur[m], ec[m]);
+      gsum[m] += (float)(ucur[m].re * g.im - ucur[m].im * g.re);
+      if (store && valid) dst[lo_n + (unsigned)(TL * m)] = g;
+    }
+  };
+  fetch(b0);
+  step(b0);
+  for (int b = b0 + 1; b < b1; ++b) step(b);
+  if (valid) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) partial[(long long)y * N + t + TL * m] = gsum[m];
   }
 }
 
@@ -868,6 +959,93 @@ __device__ __forceinline__ void small_body(const PassArgs<T>& a, int b, int mode
       for (int m = 0; m < E; ++m) dst[t + TL * m] = v[m];
     }
   }
+}
+
+// Whole field in LDS, pipelined over a group of fields [b0, b1): the next field is
+// loaded into registers while the current one runs its three in-LDS phases (rows ->
+// columns with H and pupil -> rows).  Twiddles and the fields' pointers and optics
+// come from LDS (twl, pf), so no wait in the loop covers the prefetch before it is
+// needed; the first field is peeled off the loop for the same reason as in the
+// column pass.  Complex64, evanescent modes passed through, no mask multipliers and
+// no spatial prologue/epilogue (flags within F_H, F_APERTURE, F_SCALE, F_SPEC_OUT).
+template <typename T, int N, int E, int PS>
+__device__ __forceinline__ void small_pipe_body(const PassArgs<T>& a, int b0, int b1,
+                                                cx<T>* lds, const cx<T>* twl,
+                                                const PipeField* pf, int tid) {
+  static_assert(sizeof(T) == 4, "complex64 path");
+  constexpr int TL = N / E;
+  constexpr int P = SmallPitch<N, PS>::value;
+  const int flags = a.flags;
+  const bool mid = (flags & (F_H | F_APERTURE)) != 0;
+  const bool scale = (flags & F_SCALE) != 0;
+  const bool spec_out = (flags & F_SPEC_OUT) != 0;
+  const int y = tid / TL, tr = tid % TL;  // row phases: row y, lane tr
+  const int x = tid % N, tc = tid / N;    // column phase: column x, lane tc
+  const int sx = x < N / 2 ? x : x - N;
+  cx<T> cur[E], nxt[E];
+  auto fetch = [&](int bf) {
+    const LP2D_GLB cx<T>* __restrict__ src =
+        glb(uniform_ptr((const cx<T>*)pf[bf - b0].src)) + (long long)y * N;
+#pragma unroll
+    for (int m = 0; m < E; ++m) nxt[m] = src[tr + TL * m];
+  };
+  auto step = [&](int b) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) cur[m] = nxt[m];
+    if (b + 1 < b1) fetch(b + 1);  // uniform branch
+    {  // rows: forward FFT into the tile
+      LineAcc<T, PS> acc{lds + y * P};
+      line_fft_l<T, N, E, false>(cur, tr, acc, twl);
+#pragma unroll
+      for (int m = 0; m < E; ++m) acc.st(tr + TL * m, cur[m]);
+    }
+    __syncthreads();
+    {  // columns: forward FFT, H and pupil, inverse FFT
+      StrideAcc<T, P> acc{lds + pad_idx<PS>(x)};
+      cx<T> v[E];
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = acc.ld(tc + TL * m);
+      __syncthreads();  // the column FFT below overwrites rows other threads still read
+      line_fft_l<T, N, E, false>(v, tc, acc, twl);
+      if (mid) {
+        ColMid cm = col_mid(pf[b - b0].o, flags, sx);
+        int tt = tc;
+        asm volatile("" : "+v"(cm.c0), "+v"(cm.fx2), "+v"(cm.inv_ndL), "+v"(tt));
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = mid_fast(cm, v[m], tt + TL * m - (m >= E / 2 ? N : 0));
+      }
+      if (spec_out) {
+        LP2D_GLB cx<T>* __restrict__ so = glb(a.spec_out + (long long)b * a.fstride + x);
+#pragma unroll
+        for (int m = 0; m < E; ++m) so[(long long)(tc + TL * m) * N] = v[m];
+      }
+      line_fft_l<T, N, E, true>(v, tc, acc, twl);
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < E; ++m) acc.st(tc + TL * m, v[m]);
+    }
+    __syncthreads();
+    {  // rows: inverse FFT, scale, store
+      LineAcc<T, PS> acc{lds + y * P};
+      cx<T> v[E];
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = acc.ld(tr + TL * m);
+      __syncthreads();
+      line_fft_l<T, N, E, true>(v, tr, acc, twl);
+      if (scale) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = cscale(v[m], a.scale);
+      }
+      LP2D_GLB cx<T>* __restrict__ dst =
+          glb(uniform_ptr((cx<T>*)pf[b - b0].dst)) + (long long)y * N;
+#pragma unroll
+      for (int m = 0; m < E; ++m) dst[tr + TL * m] = v[m];
+    }
+    __syncthreads();  // the next field's row phase rewrites the tile
+  };
+  fetch(b0);
+  step(b0);
+  for (int b = b0 + 1; b < b1; ++b) step(b);
 }
 
 }  // namespace lp2d

@@ -26,59 +26,92 @@ struct C2 {
 // coeffs[b, k] (complex128, pre-zeroed) += sum over a segment of p of
 // modes[k, p] * field[b, window(p)]  -- the reference's unconjugated product
 // modes_matrix @ flatten_field (beam2d.py:597), accumulated in FP64.
-// Grid: x = segments of the nb*nb pixels, y = mode k, z = group of fields.
-template <typename T>
+// One workgroup per (pixel segment, kModesPerPass modes, kFieldsPerPass fields): each
+// mode element read serves the group's fields and each field element the group's
+// modes.  Workgroups go to the 8 XCDs round robin; the linear id is decoded so that
+// each XCD takes its own segments and runs all mode groups of a segment back to back,
+// the segment of the fields staying in that XCD's L2.
+constexpr int kModesPerPass = 4;
+template <typename T, int KF>
 __global__ __launch_bounds__(kThreads) void k_project(const C2<T>* __restrict__ modes,
                                                       const C2<T>* __restrict__ field,
                                                       double2* __restrict__ coeffs, int n,
-                                                      int nb, int nmodes, int batch, int seg) {
-  __shared__ double red[kThreads / 64][2 * kFieldsPerPass];
-  const int k = blockIdx.y;
-  const int b0 = blockIdx.z * kFieldsPerPass;
-  const int nf = batch - b0 < kFieldsPerPass ? batch - b0 : kFieldsPerPass;
+                                                      int nb, int nmodes, int batch, int seg,
+                                                      int nseg) {
+  constexpr int KM = kModesPerPass;
+  __shared__ double red[kThreads / 64][2 * KM * KF];
+  const int mgroups = (nmodes + KM - 1) / KM;
+  const long long id = blockIdx.x;
+  const int xcd = (int)(id % 8);
+  const long long j = id / 8;
+  const int mg = (int)(j % mgroups);
+  const long long sg = j / mgroups;      // (segment / 8, field group) on this XCD
+  const int per_xcd = (nseg + 7) / 8;    // segment slots per XCD
+  const int grp = (int)(sg / per_xcd);
+  const int s8 = (int)(sg - (long long)grp * per_xcd);
+  const int sgi = s8 * 8 + xcd;          // segment index
+  if (sgi >= nseg) return;
+  const int k0 = mg * KM;
+  const int nk = nmodes - k0 < KM ? nmodes - k0 : KM;
+  const int b0 = grp * KF;
+  const int nf = batch - b0 < KF ? batch - b0 : KF;
   const long long npix = (long long)nb * nb;
-  const long long p0 = (long long)blockIdx.x * seg;
+  const long long p0 = (long long)sgi * seg;
   const long long p1 = p0 + seg < npix ? p0 + seg : npix;
   const int dN = (n - nb) / 2;
-  double ar[kFieldsPerPass] = {}, ai[kFieldsPerPass] = {};
-  const C2<T>* mrow = modes + (long long)k * npix;
+  double ar[KM][KF] = {}, ai[KM][KF] = {};
   for (long long p = p0 + threadIdx.x; p < p1; p += kThreads) {
-    const C2<T> m = mrow[p];
     const int jj = (int)(p / nb), ii = (int)(p - (long long)jj * nb);
     const long long off = (long long)(dN + jj) * n + (dN + ii);
+    double fx[KF], fy[KF];
 #pragma unroll
-    for (int f = 0; f < kFieldsPerPass; ++f) {
-      if (f < nf) {
-        const C2<T> u = field[(long long)(b0 + f) * n * n + off];
-        ar[f] += (double)m.x * (double)u.x - (double)m.y * (double)u.y;
-        ai[f] += (double)m.x * (double)u.y + (double)m.y * (double)u.x;
+    for (int f = 0; f < KF; ++f) {
+      C2<T> u;
+      u.x = 0;
+      u.y = 0;
+      if (f < nf) u = field[(long long)(b0 + f) * n * n + off];
+      fx[f] = (double)u.x;
+      fy[f] = (double)u.y;
+    }
+#pragma unroll
+    for (int q = 0; q < KM; ++q) {
+      C2<T> m;
+      m.x = 0;
+      m.y = 0;
+      if (q < nk) m = modes[(long long)(k0 + q) * npix + p];
+      const double mx = (double)m.x, my = (double)m.y;
+#pragma unroll
+      for (int f = 0; f < KF; ++f) {
+        ar[q][f] += mx * fx[f] - my * fy[f];
+        ai[q][f] += mx * fy[f] + my * fx[f];
       }
     }
   }
   // wave reduction, then across the block's waves
-#pragma unroll
-  for (int f = 0; f < kFieldsPerPass; ++f) {
-    for (int o = 32; o > 0; o >>= 1) {
-      ar[f] += __shfl_down(ar[f], o, 64);
-      ai[f] += __shfl_down(ai[f], o, 64);
-    }
-  }
   const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
-  if (lane == 0) {
 #pragma unroll
-    for (int f = 0; f < kFieldsPerPass; ++f) {
-      red[w][2 * f] = ar[f];
-      red[w][2 * f + 1] = ai[f];
+  for (int q = 0; q < KM; ++q) {
+#pragma unroll
+    for (int f = 0; f < KF; ++f) {
+      double r = ar[q][f], i2 = ai[q][f];
+      for (int o = 32; o > 0; o >>= 1) {
+        r += __shfl_down(r, o, 64);
+        i2 += __shfl_down(i2, o, 64);
+      }
+      if (lane == 0) {
+        red[w][2 * (q * KF + f)] = r;
+        red[w][2 * (q * KF + f) + 1] = i2;
+      }
     }
   }
   __syncthreads();
-  if (threadIdx.x < 2 * kFieldsPerPass) {
-    const int f = threadIdx.x / 2;
-    double s = 0;
-    for (int i = 0; i < kThreads / 64; ++i) s += red[i][threadIdx.x];
-    if (f < nf) {
-      double* dst = reinterpret_cast<double*>(&coeffs[(long long)(b0 + f) * nmodes + k]);
-      atomicAdd(dst + (threadIdx.x & 1), s);
+  if (threadIdx.x < 2 * KM * KF) {
+    const int qf = threadIdx.x / 2, q = qf / KF, f = qf % KF;
+    double sum = 0;
+    for (int i3 = 0; i3 < kThreads / 64; ++i3) sum += red[i3][threadIdx.x];
+    if (q < nk && f < nf) {
+      double* dst = reinterpret_cast<double*>(&coeffs[(long long)(b0 + f) * nmodes + k0 + q]);
+      atomicAdd(dst + (threadIdx.x & 1), sum);
     }
   }
 }
@@ -160,22 +193,42 @@ extern "C" int lp2d_modes_project(int n, int nb, int nmodes, int batch, int dtyp
   k_zero<<<(unsigned)((cnt + 255) / 256 < 4096 ? (cnt + 255) / 256 : 4096), 256, 0, s>>>(
       (double2*)coeffs, cnt);
   const long long npix = (long long)nb * nb;
-  const int groups = (batch + kFieldsPerPass - 1) / kFieldsPerPass;
-  // about 4096 workgroups in total, segments of at least 4 pixels per thread
-  long long nseg = 4096 / ((long long)nmodes * groups);
-  const long long maxseg = (npix + 4 * kThreads - 1) / (4 * kThreads);
+  // fields per workgroup: 8 (each mode element serves 8 fields), or 1 for one field
+  const int kf = batch > 1 ? kFieldsPerPass : 1;
+  const int groups = (batch + kf - 1) / kf;
+  const int mgroups = (nmodes + kModesPerPass - 1) / kModesPerPass;
+  // about 1024 workgroups; at least 16 pixels per thread with 8 fields (amortising
+  // the reduction of 64 sums), 4 with one
+  const int min_px = kf > 1 ? 16 : 4;
+  long long nseg = 1024 / ((long long)mgroups * groups);
+  const long long maxseg = (npix + (long long)min_px * kThreads - 1) / ((long long)min_px * kThreads);
   if (nseg > maxseg) nseg = maxseg;
   if (nseg < 1) nseg = 1;
   const int seg = (int)((npix + nseg - 1) / nseg);
   nseg = (npix + seg - 1) / seg;
-  const dim3 grid((unsigned)nseg, (unsigned)nmodes, (unsigned)groups);
-  if (dtype == LP2D_C64)
-    k_project<float><<<grid, kThreads, 0, s>>>((const C2<float>*)modes, (const C2<float>*)field,
-                                                (double2*)coeffs, n, nb, nmodes, batch, seg);
-  else
-    k_project<double><<<grid, kThreads, 0, s>>>((const C2<double>*)modes,
-                                                 (const C2<double>*)field, (double2*)coeffs, n,
-                                                 nb, nmodes, batch, seg);
+  const long long per_xcd = (nseg + 7) / 8;
+  const long long nblocks = 8 * per_xcd * mgroups * groups;
+  if (nblocks > 0x7fffffffLL) return lp2d::set_error(LP2D_E_SIZE, "too many modes x fields");
+  const unsigned nb_ = (unsigned)nblocks;
+  if (dtype == LP2D_C64) {
+    if (kf > 1)
+      k_project<float, kFieldsPerPass><<<nb_, kThreads, 0, s>>>(
+          (const C2<float>*)modes, (const C2<float>*)field, (double2*)coeffs, n, nb, nmodes,
+          batch, seg, (int)nseg);
+    else
+      k_project<float, 1><<<nb_, kThreads, 0, s>>>((const C2<float>*)modes,
+                                                   (const C2<float>*)field, (double2*)coeffs, n,
+                                                   nb, nmodes, batch, seg, (int)nseg);
+  } else {
+    if (kf > 1)
+      k_project<double, kFieldsPerPass><<<nb_, kThreads, 0, s>>>(
+          (const C2<double>*)modes, (const C2<double>*)field, (double2*)coeffs, n, nb, nmodes,
+          batch, seg, (int)nseg);
+    else
+      k_project<double, 1><<<nb_, kThreads, 0, s>>>((const C2<double>*)modes,
+                                                    (const C2<double>*)field, (double2*)coeffs, n,
+                                                    nb, nmodes, batch, seg, (int)nseg);
+  }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? LP2D_OK : lp2d::set_error(LP2D_E_HIP, hipGetErrorString(e));
 }

@@ -90,3 +90,25 @@ def test_python_layer_rejects_cpu_tensors():
         P.fft2(torch.zeros((8, 8), dtype=torch.complex64))
     with pytest.raises(Lp2dError):
         P.propagate(torch.zeros((8, 8), dtype=torch.complex64), 1e-3, 1e-2, 5e-5)
+
+
+def test_argument_errors_of_the_generator_resample_and_mode_entry_points(lib):
+    from lightprop2d_amd import _lib
+    p = C.c_void_p(16)
+    prm = _lib.lp2d_profile_params(1, 1, 1, 0.6, 0, 0, 1, 1, 1, 1, 0)
+    assert lib.lp2d_profile(99, 8, 8, p, p, C.byref(prm), None, p, 3, None) == _lib.LP2D_E_ARG
+    assert lib.lp2d_profile(1, 0, 8, p, p, C.byref(prm), None, p, 3, None) == _lib.LP2D_E_SIZE
+    assert lib.lp2d_profile(1, 8, 8, None, p, C.byref(prm), None, p, 3, None) == _lib.LP2D_E_ARG
+    # wrong output dtype for the kind (round hole -> bool)
+    assert lib.lp2d_profile(2, 8, 8, p, p, C.byref(prm), None, p, 3, None) == _lib.LP2D_E_DTYPE
+    # random kinds need the host draw
+    assert lib.lp2d_profile(5, 8, 8, p, p, C.byref(prm), None, p, 3, None) == _lib.LP2D_E_ARG
+    assert lib.lp2d_expand(64, 63, 1, 0, p, p, None) == _lib.LP2D_E_SIZE
+    assert lib.lp2d_expand(64, 129, 1, 0, p, p, None) == _lib.LP2D_E_SIZE
+    assert lib.lp2d_expand(64, 128, 1, 5, p, p, None) == _lib.LP2D_E_DTYPE
+    assert lib.lp2d_coarse(64, 3, 1, 0, p, p, None) == _lib.LP2D_E_SIZE
+    assert lib.lp2d_coarse(64, 2, 1, 0, None, p, None) == _lib.LP2D_E_ARG
+    assert lib.lp2d_modes_project(64, 65, 3, 1, 0, p, p, p, None) == _lib.LP2D_E_SIZE
+    assert lib.lp2d_modes_project(64, 63, 3, 1, 0, p, p, p, None) == _lib.LP2D_E_SIZE
+    assert lib.lp2d_modes_construct(64, 32, 0, 1, 0, p, p, p, None) == _lib.LP2D_E_SIZE
+    assert lib.lp2d_modes_construct(64, 32, 3, 1, 2, p, p, p, None) == _lib.LP2D_E_DTYPE
