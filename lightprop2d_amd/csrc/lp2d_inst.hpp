@@ -27,6 +27,18 @@ __global__ __launch_bounds__((N / E) * LINES) LP2D_OCC void k_rows(PassArgs<T> a
   rows_body<T, N, E, LINES, INV, PS, FEAT>(a, b, y0, cmin(LINES, N - y0), lds, threadIdx.x);
 }
 
+// Pipelined row pass: about one resident round of workgroups, each taking every
+// gridDim.x-th row tile of the batch; the twiddle tables are staged in LDS once.
+template <typename T, int N, int E, int LINES, bool INV, int PS, bool TILED>
+__global__ __launch_bounds__((N / E) * LINES) LP2D_OCC void k_rows_pipe(PassArgs<T> a, int nunits) {
+  __shared__ cx<T> lds[LINES * RowPitch<N, PS, LINES>::value];
+  __shared__ cx<T> twl[TwlLen<N, E>::value];
+  twl_fill<T, N, E, (N / E) * LINES>(twl, a.tw, threadIdx.x);
+  __syncthreads();
+  rows_pipe_body<T, N, E, LINES, INV, PS, TILED>(a, blockIdx.x, gridDim.x, nunits, lds, twl,
+                                                 threadIdx.x);
+}
+
 // The pipelined gradient body when its screen factor is hoisted and the LDS twiddle
 // tables fit beside the row tiles.
 template <typename T, int N, int E, int LINES, int PS>
@@ -173,6 +185,53 @@ hipError_t launch_rows(const PassArgs<T>& a, bool inv, hipStream_t s) {
   return hipGetLastError();
 }
 
+// The pipelined class covers the row passes of strided complex64 batches without
+// spatial features (the passes of a plain propagation); anything else runs the plain
+// class of the same shape.
+template <typename T, int N, int E, int LINES, int PS>
+hipError_t launch_rows_pipe(const PassArgs<T>& a, bool inv, hipStream_t s) {
+  constexpr bool TILE_OK = LINES % kTileRows == 0 && (N / E) % kTileCols == 0;
+  if ((a.flags & F_TILED) && !TILE_OK) return hipErrorInvalidValue;
+  if constexpr (sizeof(T) == 4 && N % LINES == 0) {
+    const bool feat = (a.flags & (F_SPATIAL | F_GRAD | F_INTENSITY | F_NO_STORE)) != 0;
+    const long long units = (long long)a.batch * (N / LINES);
+    if (!feat && !a.src_tab && !a.dst_tab && units <= 0x7fffffff) {
+      constexpr int threads = (N / E) * LINES;
+      const bool tiled = (a.flags & F_TILED) != 0;
+      const void* k = nullptr;
+      if constexpr (TILE_OK) {
+        if (tiled)
+          k = inv ? (const void*)&k_rows_pipe<T, N, E, LINES, true, PS, true>
+                  : (const void*)&k_rows_pipe<T, N, E, LINES, false, PS, true>;
+      }
+      if (!tiled)
+        k = inv ? (const void*)&k_rows_pipe<T, N, E, LINES, true, PS, false>
+                : (const void*)&k_rows_pipe<T, N, E, LINES, false, PS, false>;
+      // same value on every thread that races to set it
+      static int slots[4] = {0, 0, 0, 0};
+      int& sl = slots[(inv ? 2 : 0) + (tiled ? 1 : 0)];
+      if (sl == 0) sl = resident_groups(k, threads);
+      const int grid = (int)(units < sl ? units : sl);
+      const int nu = (int)units;
+      if constexpr (TILE_OK) {
+        if (tiled) {
+          if (inv)
+            k_rows_pipe<T, N, E, LINES, true, PS, true><<<grid, threads, 0, s>>>(a, nu);
+          else
+            k_rows_pipe<T, N, E, LINES, false, PS, true><<<grid, threads, 0, s>>>(a, nu);
+          return hipGetLastError();
+        }
+      }
+      if (inv)
+        k_rows_pipe<T, N, E, LINES, true, PS, false><<<grid, threads, 0, s>>>(a, nu);
+      else
+        k_rows_pipe<T, N, E, LINES, false, PS, false><<<grid, threads, 0, s>>>(a, nu);
+      return hipGetLastError();
+    }
+  }
+  return launch_rows<T, N, E, LINES, PS>(a, inv, s);
+}
+
 template <typename T, int N, int E, int LINES, int PS, bool HOIST>
 hipError_t launch_rows_grad(const PassArgs<T>& a, int groups, float* partial, hipStream_t s) {
   // reading a tiled plane in place needs whole 8-row blocks unless nothing is stored
@@ -317,7 +376,20 @@ void add_row(Registry<T>& r) {
   // a 1024-thread workgroup per CU stalls the CU at every exchange barrier)
   r.rows[lg][k++] = RowVar<T>{&launch_rows<T, N, E, S::LINES, S::PS>,
                               &launch_rows_grad<T, N, E, S::LINES, S::PS, (E <= 8)>, E,
-                              S::LINES, S::THREADS, S::LDS, S::LINES};
+                              S::LINES, S::THREADS, S::LDS, S::LINES, 0};
+}
+
+template <typename T, int N, int E>
+void add_row_pipe(Registry<T>& r) {
+  using S = RowShape<T, N, E>;
+  constexpr int lg = ilog2c(N);
+  int& k = r.nrows[lg];
+  if (k >= kMaxVariants) return;
+  constexpr int lds = S::LDS + TwlLen<N, E>::value * (int)sizeof(cx<T>);
+  if constexpr (lds <= kLdsBudget)
+    r.rows[lg][k++] = RowVar<T>{&launch_rows_pipe<T, N, E, S::LINES, S::PS>,
+                                &launch_rows_grad<T, N, E, S::LINES, S::PS, (E <= 8)>, E,
+                                S::LINES, S::THREADS, lds, S::LINES, 2};
 }
 
 template <typename T, int N, int E, int WT, bool HOIST = false>
@@ -370,6 +442,7 @@ void register_n(Registry<T>& r) {
       add_col_pipe<T, N, 32, 8>(r);
       add_col_pipe<T, N, 16, 8>(r);
       add_col_pipe<T, N, 32, 4>(r);
+      add_row_pipe<T, N, 32>(r);
     } else {
       add_row<T, N, 16>(r);
       add_col<T, N, 16, 8>(r);

@@ -124,6 +124,47 @@ __device__ __forceinline__ T* uniform_ptr(T* p) {
   return (T*)(((uint64_t)hi << 32) | lo);
 }
 
+// Streaming accesses to field planes.  Every element of a plane is read once and
+// written once per sweep, and a sweep's planes are far larger than the caches, so the
+// loads and stores may carry the nontemporal hint (global_load/store ... nt).  A plain
+// 256 MiB copy runs at 6.0 TB/s without the hint and 6.4 TB/s with it on MI355X, but
+// the hint changes how the next sweep finds the plane, so it is chosen per access
+// site by measurement.  LP2D_NT is a mask of the sites that carry it.  Measured at
+// 32 x 1024^2 complex64: the row-pass loads gain 5-7 us per row pass and 8 us in the
+// following column pass, the gradient pass 28 us; hinted stores and column loads
+// were slower in every combination, the whole-field kernels indifferent.
+enum : int {
+  NT_ROW_LD = 1, NT_ROW_ST = 2,    // row passes
+  NT_COL_LD = 4, NT_COL_ST = 8,    // column passes
+  NT_GRAD = 16,                    // adjoint gradient row pass
+  NT_SMALL = 32,                   // whole-field kernels
+};
+#ifndef LP2D_NT
+#define LP2D_NT (NT_ROW_LD | NT_GRAD)
+#endif
+template <int SITE, typename T>
+__device__ __forceinline__ cx<T> gld(const LP2D_GLB cx<T>* p) {
+  if constexpr ((LP2D_NT & SITE) != 0) {
+    typedef T V __attribute__((ext_vector_type(2)));
+    const V v = __builtin_nontemporal_load((const LP2D_GLB V*)p);
+    return mk(v.x, v.y);
+  } else {
+    return *p;
+  }
+}
+template <int SITE, typename T>
+__device__ __forceinline__ void gst(LP2D_GLB cx<T>* p, cx<T> x) {
+  if constexpr ((LP2D_NT & SITE) != 0) {
+    typedef T V __attribute__((ext_vector_type(2)));
+    V v;
+    v.x = x.re;
+    v.y = x.im;
+    __builtin_nontemporal_store(v, (LP2D_GLB V*)p);
+  } else {
+    *p = x;
+  }
+}
+
 // Base of field b's source / destination plane (b uniform across the workgroup).
 template <typename T>
 __device__ __forceinline__ const LP2D_GLB cx<T>* field_src(const PassArgs<T>& a, int b) {
@@ -468,10 +509,10 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
   const LP2D_GLB cx<T>* __restrict__ src = field_src(a, b) + ubase;
   if (INV && tiled) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = src[lo_t + (unsigned)(kTileRows * TL * m)];
+    for (int m = 0; m < E; ++m) v[m] = gld<NT_ROW_LD>(src + (lo_t + (unsigned)(kTileRows * TL * m)));
   } else {
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = src[lo + (unsigned)(TL * m)];
+    for (int m = 0; m < E; ++m) v[m] = gld<NT_ROW_LD>(src + (lo + (unsigned)(TL * m)));
   }
   if constexpr (FEAT) {
     if (a.flags & F_SPATIAL) {
@@ -491,11 +532,11 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
     if ((a.flags & F_NO_STORE) == 0) {
       if (!INV && tiled) {
 #pragma unroll
-        for (int m = 0; m < E; ++m) dst[lo_t + (unsigned)(kTileRows * TL * m)] = v[m];
+        for (int m = 0; m < E; ++m) gst<NT_ROW_ST>(dst + (lo_t + (unsigned)(kTileRows * TL * m)), v[m]);
       } else {
 #pragma unroll
         for (int m = 0; m < E; ++m)
-          if (valid) dst[lo + (unsigned)(TL * m)] = v[m];
+          if (valid) gst<NT_ROW_ST>(dst + (lo + (unsigned)(TL * m)), v[m]);
       }
     }
   } else {
@@ -503,13 +544,89 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
     if (!INV && tiled) {
 #pragma unroll
       for (int m = 0; m < E; ++m)
-        dst[lo_t + (unsigned)(kTileRows * TL * m)] = mk(v[m].re * sc, v[m].im * sc);
+        gst<NT_ROW_ST>(dst + (lo_t + (unsigned)(kTileRows * TL * m)), mk(v[m].re * sc, v[m].im * sc));
     } else {
 #pragma unroll
       for (int m = 0; m < E; ++m)
-        if (valid) dst[lo + (unsigned)(TL * m)] = mk(v[m].re * sc, v[m].im * sc);
+        if (valid) gst<NT_ROW_ST>(dst + (lo + (unsigned)(TL * m)), mk(v[m].re * sc, v[m].im * sc));
     }
   }
+}
+
+// Row tiles of LINES rows with software pipelining: the tile of step k+1 is loaded
+// into registers while tile k is transformed, as in the pipelined column pass.  A
+// workgroup takes tiles u0, u0 + ustep, u0 + 2*ustep, ... of the whole batch, so the
+// resident workgroups sweep the planes together.  Twiddles come from LDS (twl), so
+// nothing in the loop waits for the prefetch before it is needed.  Strided batches
+// of complex64 without spatial features; TILED: the forward pass stores, the inverse
+// pass loads the block-tiled layout.
+template <typename T, int N, int E, int LINES, bool INV, int PS, bool TILED>
+__device__ __forceinline__ void rows_pipe_body(const PassArgs<T>& a, int u0, int ustep,
+                                               int nunits, cx<T>* lds, const cx<T>* twl,
+                                               int tid) {
+  static_assert(sizeof(T) == 4, "complex64 path");
+  constexpr int TL = N / E;
+  constexpr int L = RowPitch<N, PS, LINES>::value;
+  constexpr int TPF = N / LINES;  // row tiles per field
+  static_assert(N % LINES == 0, "whole tiles");
+  static_assert(!TILED || (LINES % kTileRows == 0 && TL % kTileCols == 0), "tiled shape");
+  int l = tid / TL, t = tid - (tid / TL) * TL;
+  if constexpr (INV && TILED) {
+    const int g = tid / (kTileRows * TL), r = tid - g * (kTileRows * TL);
+    if constexpr (TL % 16 == 0) {
+      // each wave takes 4 rows x 16 lanes of an 8-row block: every load instruction
+      // reads two 256-byte runs of the tiled plane, every store writes whole 128-byte
+      // lines of the natural one (with 8 x 8 lanes per wave the two halves of a line
+      // come from two waves, and the line is written back twice)
+      const int wv = r >> 6, ln = r & 63;
+      l = g * kTileRows + (ln >> 4) + 4 * (wv & 1);
+      t = (ln & 15) + 16 * (wv >> 1);
+    } else {
+      // 8 rows x 8 lanes: every load instruction reads one 8x8 tile
+      l = g * kTileRows + ((r & 63) >> 3);
+      t = (r >> 6) * kTileCols + (r & 7);
+    }
+  }
+  // lane offsets inside a tile of LINES rows: natural (y, x) at y*N + x, tiled see
+  // F_TILED; element m of the thread is x = t + TL*m
+  const unsigned lo_n = (unsigned)(l * N + t);
+  const unsigned lo_t = (unsigned)((l >> 3) * kTileRows * N + (t >> 3) * (kTileRows * kTileCols) +
+                                   (l & 7) * kTileCols + (t & 7));
+  constexpr bool LD_T = INV && TILED, ST_T = !INV && TILED;
+  const unsigned ld_o = LD_T ? lo_t : lo_n, st_o = ST_T ? lo_t : lo_n;
+  constexpr unsigned LD_S = LD_T ? kTileRows * TL : TL, ST_S = ST_T ? kTileRows * TL : TL;
+  LineAcc<T, PS> acc{lds + l * L};
+  const bool scale = (a.flags & F_SCALE) != 0;
+  auto tile_off = [&](int u) {
+    const int b = u / TPF;
+    return (long long)b * a.fstride + (long long)(u - b * TPF) * LINES * N;
+  };
+  // Step u moves tile u (prefetched by the previous step) from nxt into cur, requests
+  // tile u + ustep into nxt, then transforms cur and stores it; the first step is
+  // peeled so that the loop head's wait only ever covers the prefetch.
+  cx<T> cur[E], nxt[E];
+  auto fetch = [&](int u) {
+    const LP2D_GLB cx<T>* __restrict__ src = glb(uniform_ptr(a.src + tile_off(u)));
+#pragma unroll
+    for (int m = 0; m < E; ++m) nxt[m] = gld<NT_ROW_LD>(src + (ld_o + LD_S * (unsigned)m));
+  };
+  auto step = [&](int u) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) cur[m] = nxt[m];
+    if (u + ustep < nunits) fetch(u + ustep);  // uniform branch
+    line_fft_l<T, N, E, INV>(cur, t, acc, twl);
+    if (scale) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) cur[m] = cscale(cur[m], a.scale);
+    }
+    LP2D_GLB cx<T>* __restrict__ dst = glb(uniform_ptr(a.dst + tile_off(u)));
+#pragma unroll
+    for (int m = 0; m < E; ++m) gst<NT_ROW_ST>(dst + (st_o + ST_S * (unsigned)m), cur[m]);
+  };
+  if (u0 >= nunits) return;
+  fetch(u0);
+  step(u0);
+  for (int u = u0 + ustep; u < nunits; u += ustep) step(u);
 }
 
 // Last row pass of the adjoint for fields [b0, b1) that share one phase screen: the
@@ -549,7 +666,7 @@ __device__ __forceinline__ void rows_grad_body(const PassArgs<T>& a, int b0, int
     if (tiled) {
       const LP2D_GLB cx<T>* __restrict__ src = glb(a.src + (long long)b * a.fstride);
 #pragma unroll
-      for (int m = 0; m < E; ++m) v[m] = src[lo_t + (unsigned)(kTileRows * TL * m)];
+      for (int m = 0; m < E; ++m) v[m] = gld<NT_GRAD>(src + (lo_t + (unsigned)(kTileRows * TL * m)));
     } else {
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = a.src[frow + t + TL * m];
@@ -615,7 +732,7 @@ __device__ __forceinline__ void cols_body(const PassArgs<T>& a, int b0, int b1, 
     const LP2D_GLB cx<T>* __restrict__ src =
         (MIDK == 3 ? glb(a.src + (long long)b * a.fstride) : field_src(a, b)) + x0;
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = src[lo + (unsigned)(TL * N * m)];
+    for (int m = 0; m < E; ++m) v[m] = gld<NT_COL_LD>(src + (lo + (unsigned)(TL * N * m)));
     if constexpr ((MODE & MODE_FWD) != 0) line_fft<T, N, E, false>(v, t, acc, a.tw + (N - 2));
     if constexpr (MIDK == 3) {
       if (mid) {
@@ -649,7 +766,7 @@ __device__ __forceinline__ void cols_body(const PassArgs<T>& a, int b0, int b1, 
         (MIDK == 3 ? glb(a.dst + (long long)b * a.fstride) : field_dst(a, b)) + x0;
     const T sc = (a.flags & F_SCALE) ? a.scale : (T)1;
 #pragma unroll
-    for (int m = 0; m < E; ++m) dst[lo + (unsigned)(TL * N * m)] = mk(v[m].re * sc, v[m].im * sc);
+    for (int m = 0; m < E; ++m) gst<NT_COL_ST>(dst + (lo + (unsigned)(TL * N * m)), mk(v[m].re * sc, v[m].im * sc));
   }
 }
 
@@ -686,13 +803,13 @@ __device__ __forceinline__ void rows_grad_pipe_body(const PassArgs<T>& a, int b0
     const LP2D_GLB cx<T>* __restrict__ us = glb(a.u_saved + (long long)bf * a.u_sb);
     if (tiled) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) nxt[m] = src[lo_t + (unsigned)(kTileRows * TL * m)];
+      for (int m = 0; m < E; ++m) nxt[m] = gld<NT_GRAD>(src + (lo_t + (unsigned)(kTileRows * TL * m)));
     } else {
 #pragma unroll
-      for (int m = 0; m < E; ++m) nxt[m] = src[lo_n + (unsigned)(TL * m)];
+      for (int m = 0; m < E; ++m) nxt[m] = gld<NT_GRAD>(src + (lo_n + (unsigned)(TL * m)));
     }
 #pragma unroll
-    for (int m = 0; m < E; ++m) unxt[m] = us[lo_n + (unsigned)(TL * m)];
+    for (int m = 0; m < E; ++m) unxt[m] = gld<NT_GRAD>(us + (lo_n + (unsigned)(TL * m)));
   };
   auto step = [&](int b) {
 #pragma unroll
@@ -707,7 +824,7 @@ __device__ __forceinline__ void rows_grad_pipe_body(const PassArgs<T>& a, int b0
     for (int m = 0; m < E; ++m) {
       const cx<T> g = cmul(cur[m], ec[m]);
       gsum[m] += (float)(ucur[m].re * g.im - ucur[m].im * g.re);
-      if (store && valid) dst[lo_n + (unsigned)(TL * m)] = g;
+      if (store && valid) gst<NT_GRAD>(dst + (lo_n + (unsigned)(TL * m)), g);
     }
   };
   fetch(b0);
@@ -843,7 +960,7 @@ __device__ __forceinline__ void cols_pipe_body(const PassArgs<T>& a, int b0, int
   auto fetch = [&](int bf) {
     const LP2D_GLB cx<T>* __restrict__ src = glb(uniform_ptr((const cx<T>*)pf[bf - b0].src)) + xoff;
 #pragma unroll
-    for (int m = 0; m < E; ++m) nxt[m] = src[lo + (unsigned)(TL * N * m)];
+    for (int m = 0; m < E; ++m) nxt[m] = gld<NT_COL_LD>(src + (lo + (unsigned)(TL * N * m)));
   };
   auto step = [&](int b) {
 #pragma unroll
@@ -864,7 +981,7 @@ __device__ __forceinline__ void cols_pipe_body(const PassArgs<T>& a, int b0, int
     if (spec_out) {
       LP2D_GLB cx<T>* __restrict__ so = glb(a.spec_out + (long long)b * a.fstride + x0);
 #pragma unroll
-      for (int m = 0; m < E; ++m) so[lo_n + (unsigned)(TL * N * m)] = cur[m];
+      for (int m = 0; m < E; ++m) gst<NT_COL_ST>(so + (lo_n + (unsigned)(TL * N * m)), cur[m]);
     }
     line_fft_l<T, N, E, true>(cur, t, acc, twl);
     if (scale) {
@@ -873,7 +990,7 @@ __device__ __forceinline__ void cols_pipe_body(const PassArgs<T>& a, int b0, int
     }
     LP2D_GLB cx<T>* __restrict__ dst = glb(uniform_ptr((cx<T>*)pf[b - b0].dst)) + xoff;
 #pragma unroll
-    for (int m = 0; m < E; ++m) dst[lo + (unsigned)(TL * N * m)] = cur[m];
+    for (int m = 0; m < E; ++m) gst<NT_COL_ST>(dst + (lo + (unsigned)(TL * N * m)), cur[m]);
   };
   fetch(b0);
   step(b0);
@@ -905,7 +1022,7 @@ __device__ __forceinline__ void small_body(const PassArgs<T>& a, int b, int mode
     const int y = tid / TL, t = tid % TL;
     const LP2D_GLB cx<T>* __restrict__ src = fsrc + (long long)y * N;
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = src[t + TL * m];
+    for (int m = 0; m < E; ++m) v[m] = gld<NT_SMALL>(src + (t + TL * m));
     if (a.flags & F_SPATIAL) {
       spatial_pro_all<T, E>(a, v, b, y, t, TL, N);
       if (a.flags & F_FIELD_OUT) {
@@ -956,7 +1073,7 @@ __device__ __forceinline__ void small_body(const PassArgs<T>& a, int b, int mode
     spatial_epi_all<T, E>(a, v, b, y, t, TL, N, fb + (long long)y * N + t, true);
     if ((a.flags & F_NO_STORE) == 0) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) dst[t + TL * m] = v[m];
+      for (int m = 0; m < E; ++m) gst<NT_SMALL>(dst + (t + TL * m), v[m]);
     }
   }
 }
@@ -987,7 +1104,7 @@ __device__ __forceinline__ void small_pipe_body(const PassArgs<T>& a, int b0, in
     const LP2D_GLB cx<T>* __restrict__ src =
         glb(uniform_ptr((const cx<T>*)pf[bf - b0].src)) + (long long)y * N;
 #pragma unroll
-    for (int m = 0; m < E; ++m) nxt[m] = src[tr + TL * m];
+    for (int m = 0; m < E; ++m) nxt[m] = gld<NT_SMALL>(src + (tr + TL * m));
   };
   auto step = [&](int b) {
 #pragma unroll
@@ -1017,7 +1134,7 @@ __device__ __forceinline__ void small_pipe_body(const PassArgs<T>& a, int b0, in
       if (spec_out) {
         LP2D_GLB cx<T>* __restrict__ so = glb(a.spec_out + (long long)b * a.fstride + x);
 #pragma unroll
-        for (int m = 0; m < E; ++m) so[(long long)(tc + TL * m) * N] = v[m];
+        for (int m = 0; m < E; ++m) gst<NT_SMALL>(so + ((long long)(tc + TL * m) * N), v[m]);
       }
       line_fft_l<T, N, E, true>(v, tc, acc, twl);
       __syncthreads();
@@ -1039,7 +1156,7 @@ __device__ __forceinline__ void small_pipe_body(const PassArgs<T>& a, int b0, in
       LP2D_GLB cx<T>* __restrict__ dst =
           glb(uniform_ptr((cx<T>*)pf[b - b0].dst)) + (long long)y * N;
 #pragma unroll
-      for (int m = 0; m < E; ++m) dst[tr + TL * m] = v[m];
+      for (int m = 0; m < E; ++m) gst<NT_SMALL>(dst + (tr + TL * m), v[m]);
     }
     __syncthreads();  // the next field's row phase rewrites the tile
   };

@@ -25,6 +25,7 @@ struct RowVar {
   RowGradLaunch<T> grad_fn;  // adjoint last pass with the batch-summed phase gradient
   int E, lines, threads, lds;
   int grad_lines;  // row tile of grad_fn: whole 8-row blocks where the threads allow
+  int kind;        // bit 1: software-pipelined over the row tiles of the batch
 };
 template <typename T>
 struct ColVar {

@@ -117,7 +117,7 @@ def test_every_column_variant_on_uneven_tables(restore_selection, dtype):
         lib.lp2d_set_variant(n, dt, 1, default)
 
 
-@pytest.mark.parametrize("n,batch", [(128, 8300), (512, 300), (1024, 72), (256, 1)])
+@pytest.mark.parametrize("n,batch", [(128, 8300), (512, 300), (1024, 72), (256, 200), (256, 1)])
 def test_pipelined_groups_at_large_and_single_batches(n, batch):
     """Batches that make the pipelined kernels cap their groups at 32 fields per
     workgroup (8300 x 128^2, 300 x 512^2), a partial last group (72 x 1024^2) and one
