@@ -208,6 +208,16 @@ int lp2d_propagate_adjoint(int n, int batch, int dtype, const void* grad_out,
                            const void* field_in, const float* phase, int64_t phase_stride_b,
                            const lp2d_optics* optics, void* grad_in, float* grad_phase,
                            int adj_flags, void* stream);
+/* lp2d_propagate_adjoint with a workspace: `work` (batch x n x n of `dtype`, or NULL
+ * for the call above) receives the column pass's output, so the last pass reads one
+ * buffer and writes another.  The intermediate plane may then stay block-tiled for
+ * every gradient schedule (in place, the gradient pass would have to convert whole
+ * 8-row blocks).  work must not alias grad_out, field_in or grad_in; its contents are
+ * undefined on return.  No counterpart in the reference (it has no gradients). */
+int lp2d_propagate_adjoint_ws(int n, int batch, int dtype, const void* grad_out,
+                              const void* field_in, const float* phase, int64_t phase_stride_b,
+                              const lp2d_optics* optics, void* grad_in, float* grad_phase,
+                              int adj_flags, void* work, void* stream);
 
 /* Sum `batch` float planes of `count` elements: out[i] = sum_b in[b*count + i].
  * Used to reduce per-field phase gradients of a shared mask. */

@@ -666,6 +666,10 @@ int run_rows_grad(const PassArgs<T>& a, int lg, float* gphase, hipStream_t s) {
   int groups = (2048 * 256 / ((n / v.E) * v.grad_lines)) / tpf;
   if (groups < 1) groups = 1;
   if (groups > a.batch) groups = a.batch;
+  // the kernel gives group g the fields [g*per, min(batch, (g+1)*per)): recount the
+  // groups so that none is empty (batch 5 in 4 groups of 2 would leave group 3 at 6)
+  const int per = (a.batch + groups - 1) / groups;
+  groups = (a.batch + per - 1) / per;
   std::string nm;
   if (g_trace_on) {
     char b[64];
@@ -689,7 +693,7 @@ int run_rows_grad(const PassArgs<T>& a, int lg, float* gphase, hipStream_t s) {
 template <typename T>
 int op_adjoint(int n, int lg, int batch, const void* gout, const void* u, const float* phase,
                int64_t phase_sb, const lp2d_optics* o, void* gin, float* gphase, int aflags,
-               hipStream_t s) {
+               void* work, hipStream_t s) {
   DeviceTables* t;
   if (int rc = current_tables(&t)) return rc;
   PassArgs<T> a = base_args<T>(t, n, batch);
@@ -717,12 +721,20 @@ int op_adjoint(int n, int lg, int batch, const void* gout, const void* u, const 
     if (int rc = run_small<T>(a, lg, MODE_FWD | MODE_INV, s)) return rc;
   } else {
     a.flags = mid;
-    const int tl = tiled_flag<T>(a, lg, shared_sum, no_store == 0);
+    // With a workspace the column pass writes there and the last pass reads it: no
+    // in-place conversion of a tiled plane, so any gradient schedule can read one.
+    int tl = tiled_flag<T>(a, lg, shared_sum, no_store == 0 && !work);
+    const bool oop = work && tl;
     a.flags = tl;
     if (int rc = run_rows<T>(a, lg, false, s)) return rc;
     a.src = (const cx<T>*)gin;
+    if (oop) a.dst = (cx<T>*)work;
     a.flags = mid | tl;
     if (int rc = run_cols<T>(a, lg, MODE_FWD | MODE_INV, s)) return rc;
+    if (oop) {
+      a.src = (const cx<T>*)work;
+      a.dst = (cx<T>*)gin;
+    }
     if (shared_sum) {
       a.flags = F_SCALE | no_store | tl;
       return run_rows_grad<T>(a, lg, gphase, s);
@@ -912,6 +924,14 @@ int lp2d_propagate(int n, int batch, int dtype, const void* field_in, void* fiel
 int lp2d_propagate_adjoint(int n, int batch, int dtype, const void* grad_out, const void* field_in,
                            const float* phase, int64_t phase_stride_b, const lp2d_optics* optics,
                            void* grad_in, float* grad_phase, int adj_flags, void* stream) {
+  return lp2d_propagate_adjoint_ws(n, batch, dtype, grad_out, field_in, phase, phase_stride_b,
+                                   optics, grad_in, grad_phase, adj_flags, nullptr, stream);
+}
+
+int lp2d_propagate_adjoint_ws(int n, int batch, int dtype, const void* grad_out,
+                              const void* field_in, const float* phase, int64_t phase_stride_b,
+                              const lp2d_optics* optics, void* grad_in, float* grad_phase,
+                              int adj_flags, void* work, void* stream) {
   int lg;
   if (int rc = check_n(n, batch, dtype, &lg)) return rc;
   if (!grad_out || !grad_in || !optics) return fail(LP2D_E_ARG, "null pointer");
@@ -921,10 +941,12 @@ int lp2d_propagate_adjoint(int n, int batch, int dtype, const void* grad_out, co
     return fail(LP2D_E_ARG, "field_in is required with a phase screen or lens");
   if (field_in && grad_in == field_in) return fail(LP2D_E_ARG, "grad_in must not alias field_in");
   if (adj_flags & ~LP2D_ADJ_NO_GRAD_IN) return fail(LP2D_E_ARG, "unknown adj_flags bits");
+  if (work && (work == grad_in || work == grad_out || work == field_in))
+    return fail(LP2D_E_ARG, "work must not alias grad_out, field_in or grad_in");
   hipStream_t s = (hipStream_t)stream;
   return dtype == LP2D_C64
-             ? op_adjoint<float>(n, lg, batch, grad_out, field_in, phase, phase_stride_b, optics, grad_in, grad_phase, adj_flags, s)
-             : op_adjoint<double>(n, lg, batch, grad_out, field_in, phase, phase_stride_b, optics, grad_in, grad_phase, adj_flags, s);
+             ? op_adjoint<float>(n, lg, batch, grad_out, field_in, phase, phase_stride_b, optics, grad_in, grad_phase, adj_flags, work, s)
+             : op_adjoint<double>(n, lg, batch, grad_out, field_in, phase, phase_stride_b, optics, grad_in, grad_phase, adj_flags, work, s);
 }
 
 int lp2d_batch_create(const lp2d_field_desc* descs, int count, int dtype, lp2d_batch** out) {

@@ -235,8 +235,10 @@ hipError_t launch_rows_pipe(const PassArgs<T>& a, bool inv, hipStream_t s) {
 template <typename T, int N, int E, int LINES, int PS, bool HOIST>
 hipError_t launch_rows_grad(const PassArgs<T>& a, int groups, float* partial, hipStream_t s) {
   // reading a tiled plane in place needs whole 8-row blocks unless nothing is stored
+  const bool in_place = (const void*)a.src == (const void*)a.dst;
   if ((a.flags & F_TILED) &&
-      (((LINES % kTileRows) != 0 && !(a.flags & F_NO_STORE)) || (N / E) % kTileCols != 0))
+      (((LINES % kTileRows) != 0 && !(a.flags & F_NO_STORE) && in_place) ||
+       (N / E) % kTileCols != 0))
     return hipErrorInvalidValue;
   constexpr int threads = (N / E) * LINES;
   constexpr int TPF = (N + LINES - 1) / LINES;

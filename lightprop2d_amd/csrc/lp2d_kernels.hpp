@@ -629,6 +629,24 @@ __device__ __forceinline__ void rows_pipe_body(const PassArgs<T>& a, int u0, int
   for (int u = u0 + ustep; u < nunits; u += ustep) step(u);
 }
 
+// Thread -> (row l, lane t) of a gradient row tile of LINES rows.  Reading a
+// block-tiled plane, each wave takes all LINES rows x 64/LINES lanes, so every load
+// instruction reads whole 128-byte lines (rows y, y+1 of an 8x8 tile are adjacent);
+// with one row per wave each line would be read in two halves by two waves.
+template <int TL, int LINES>
+__device__ __forceinline__ void grad_thread_map(int tid, bool tiled, int& l, int& t) {
+  l = tid / TL;
+  t = tid - (tid / TL) * TL;
+  if constexpr (64 % LINES == 0 && TL % (64 / LINES) == 0 && LINES * TL >= 64) {
+    if (tiled) {
+      constexpr int LPW = 64 / LINES;  // lanes per row in one wave
+      const int lane = tid & 63, w = tid >> 6;
+      l = lane / LPW;
+      t = w * LPW + (lane - (lane / LPW) * LPW);
+    }
+  }
+}
+
 // Last row pass of the adjoint for fields [b0, b1) that share one phase screen: the
 // inverse row FFT and gradient fold per field, with the phase gradient of every pixel
 // summed over the fields in registers and written once to `partial` (n x n).  The
@@ -643,7 +661,9 @@ __device__ __forceinline__ void rows_grad_body(const PassArgs<T>& a, int b0, int
                                                cx<T>* lds, int tid) {
   constexpr int TL = N / E;
   constexpr int L = PaddedLen<N, PS>::value;
-  const int l = tid / TL, t = tid - (tid / TL) * TL;
+  const bool tiled = (a.flags & F_TILED) != 0;
+  int l, t;
+  grad_thread_map<TL, LINES>(tid, tiled, l, t);
   const int y = valid_row(y0 + l, y0 + nrows - 1);
   const bool valid = l < nrows;
   const bool store = (a.flags & F_NO_STORE) == 0;
@@ -657,7 +677,6 @@ __device__ __forceinline__ void rows_grad_body(const PassArgs<T>& a, int b0, int
   }
   // block-tiled offset of (y, x = t + TL*m) in the field: lo_t + 8*TL*m (TL a multiple
   // of 8); any row tile height can read it, storing in place needs whole blocks
-  const bool tiled = (a.flags & F_TILED) != 0;
   const unsigned lo_t = (unsigned)((y >> 3) * kTileRows * N + (t >> 3) * (kTileRows * kTileCols) +
                                    (y & 7) * kTileCols + (t & 7));
   for (int b = b0; b < b1; ++b) {
@@ -781,11 +800,12 @@ __device__ __forceinline__ void rows_grad_pipe_body(const PassArgs<T>& a, int b0
                                                     cx<T>* lds, const cx<T>* twl, int tid) {
   constexpr int TL = N / E;
   constexpr int L = PaddedLen<N, PS>::value;
-  const int l = tid / TL, t = tid - (tid / TL) * TL;
+  const bool tiled = (a.flags & F_TILED) != 0;
+  int l, t;
+  grad_thread_map<TL, LINES>(tid, tiled, l, t);
   const int y = valid_row(y0 + l, y0 + nrows - 1);
   const bool valid = l < nrows;
   const bool store = (a.flags & F_NO_STORE) == 0;
-  const bool tiled = (a.flags & F_TILED) != 0;
   LineAcc<T, PS> acc{lds + l * L};
   cx<T> ec[E];
   float gsum[E];
@@ -827,9 +847,11 @@ __device__ __forceinline__ void rows_grad_pipe_body(const PassArgs<T>& a, int b0
       if (store && valid) gst<NT_GRAD>(dst + (lo_n + (unsigned)(TL * m)), g);
     }
   };
-  fetch(b0);
-  step(b0);
-  for (int b = b0 + 1; b < b1; ++b) step(b);
+  if (b0 < b1) {  // an empty group only writes its zero partial sums
+    fetch(b0);
+    step(b0);
+    for (int b = b0 + 1; b < b1; ++b) step(b);
+  }
   if (valid) {
 #pragma unroll
     for (int m = 0; m < E; ++m) partial[(long long)y * N + t + TL * m] = gsum[m];
@@ -992,6 +1014,7 @@ __device__ __forceinline__ void cols_pipe_body(const PassArgs<T>& a, int b0, int
 #pragma unroll
     for (int m = 0; m < E; ++m) gst<NT_COL_ST>(dst + (lo + (unsigned)(TL * N * m)), cur[m]);
   };
+  if (b0 >= b1) return;  // (the launchers size the groups so that none is empty)
   fetch(b0);
   step(b0);
   for (int b = b0 + 1; b < b1; ++b) step(b);
@@ -1160,6 +1183,7 @@ __device__ __forceinline__ void small_pipe_body(const PassArgs<T>& a, int b0, in
     }
     __syncthreads();  // the next field's row phase rewrites the tile
   };
+  if (b0 >= b1) return;  // (the launchers size the groups so that none is empty)
   fetch(b0);
   step(b0);
   for (int b = b0 + 1; b < b1; ++b) step(b);

@@ -229,17 +229,29 @@ def propagate_adjoint(grad_out: torch.Tensor, field_in: Optional[torch.Tensor], 
                       area_size: float, wl: float, phase: Optional[torch.Tensor] = None,
                       aperture_radius: float = 0.0, lens_f: float = 0.0,
                       evanescent: str = "reference", want_phase_grad: bool = True,
-                      want_field_grad: bool = True, out: Optional[torch.Tensor] = None):
+                      want_field_grad: bool = True, out: Optional[torch.Tensor] = None,
+                      work: Optional[torch.Tensor] = None, use_workspace: bool = True):
     """Adjoint of `propagate` (vector-Jacobian product, PyTorch complex convention).
 
     Returns (grad_field, grad_phase).  grad_phase has the phase's shape: per field for a
     (batch, n, n) phase, summed over the batch (inside the kernels) for a shared (n, n)
     screen; None when not requested.  grad_field is None when want_field_grad is False
-    (its buffer is then only the passes' workspace)."""
+    (its buffer is then only the passes' workspace).
+
+    For complex64 fields that take the three-sweep path, the column pass writes into a
+    workspace the size of grad_out (`work`, allocated here when not given), so the
+    intermediate plane can stay block-tiled; use_workspace=False keeps every pass in
+    `out` (less memory, row-major intermediate)."""
     lib = _lib()
     n, batch, dt = _field_geom(grad_out, "grad_out")
     if out is None:
         out = torch.empty_like(grad_out)
+    if work is not None:
+        if work.shape != grad_out.shape or work.dtype != grad_out.dtype or \
+                work.device != grad_out.device or not work.is_contiguous():
+            raise Lp2dError("work must be a contiguous tensor like grad_out")
+    elif use_workspace and grad_out.dtype == torch.complex64 and not _small_path(n, "c64"):
+        work = torch.empty_like(grad_out)
     ph, psb = _phase(phase, n, batch, grad_out.device)
     gph = None
     if ph is not None and want_phase_grad:
@@ -250,9 +262,10 @@ def propagate_adjoint(grad_out: torch.Tensor, field_in: Optional[torch.Tensor], 
                evanescent=evanescent)
     flags = 0 if want_field_grad else L.LP2D_ADJ_NO_GRAD_IN
     with torch.cuda.device(grad_out.device):
-        L.check(lib.lp2d_propagate_adjoint(n, batch, dt, _ptr(grad_out), _ptr(field_in),
-                                           _ptr(ph), psb, C.byref(o), _ptr(out), _ptr(gph),
-                                           flags, _stream(grad_out)), "propagate_adjoint")
+        L.check(lib.lp2d_propagate_adjoint_ws(n, batch, dt, _ptr(grad_out), _ptr(field_in),
+                                              _ptr(ph), psb, C.byref(o), _ptr(out), _ptr(gph),
+                                              flags, _ptr(work), _stream(grad_out)),
+                "propagate_adjoint")
     if gph is not None and psb == 0:
         gph = gph[0]
     return (out if want_field_grad else None), gph

@@ -68,6 +68,13 @@ def test_argument_errors_before_any_device_work(lib):
                                       None) == _lib.LP2D_E_ARG
     assert lib.lp2d_propagate_adjoint(64, 1, 0, p, p, None, 0, C.byref(o), C.c_void_p(32), None, 6,
                                       None) == _lib.LP2D_E_ARG
+    # the workspace must not alias grad_out / field_in / grad_in
+    q = C.c_void_p(48)
+    assert lib.lp2d_propagate_adjoint_ws(64, 1, 0, p, None, None, 0, C.byref(o), q, None, 0, q,
+                                         None) == _lib.LP2D_E_ARG
+    assert lib.lp2d_propagate_adjoint_ws(64, 1, 0, p, None, None, 0, C.byref(o), q, None, 0, p,
+                                         None) == _lib.LP2D_E_ARG
+    assert b"work must not alias" in lib.lp2d_last_error()
     m = (_lib.lp2d_mask * 1)(_lib.lp2d_mask(16, 42, 0, 0, 0, 0))
     assert lib.lp2d_spatial_apply(64, 1, 0, p, m, 1, None, None, 0, None, None) == _lib.LP2D_E_DTYPE
     assert lib.lp2d_spectral_apply(64, 1, 0, p, m, 3, None, None, None) == _lib.LP2D_E_ARG
