@@ -2,6 +2,8 @@
 // per-precision translation units (lp2d_f32_*.hip, lp2d_f64_*.hip) so the large set
 // of instantiations compiles in parallel.
 #pragma once
+#include <atomic>
+
 #include "lp2d_registry.hpp"
 
 namespace lp2d {
@@ -163,6 +165,17 @@ struct SmallShape {
 };
 
 // ----------------------------------------------------------------- launchers
+// Workgroups of `kernel` resident at once on the current device: measured on first
+// use and kept per kernel (every caller computes the same value).
+inline int cached_slots(std::atomic<int>& slot, const void* kernel, int threads) {
+  int v = slot.load(std::memory_order_relaxed);
+  if (v == 0) {
+    v = resident_groups(kernel, threads);
+    slot.store(v, std::memory_order_relaxed);
+  }
+  return v;
+}
+
 template <typename T, int N, int E, int LINES, int PS>
 hipError_t launch_rows(const PassArgs<T>& a, bool inv, hipStream_t s) {
   if ((a.flags & F_TILED) && (LINES % kTileRows != 0 || (N / E) % kTileCols != 0))
@@ -208,9 +221,8 @@ hipError_t launch_rows_pipe(const PassArgs<T>& a, bool inv, hipStream_t s) {
         k = inv ? (const void*)&k_rows_pipe<T, N, E, LINES, true, PS, false>
                 : (const void*)&k_rows_pipe<T, N, E, LINES, false, PS, false>;
       // same value on every thread that races to set it
-      static int slots[4] = {0, 0, 0, 0};
-      int& sl = slots[(inv ? 2 : 0) + (tiled ? 1 : 0)];
-      if (sl == 0) sl = resident_groups(k, threads);
+      static std::atomic<int> slots[4];
+      const int sl = cached_slots(slots[(inv ? 2 : 0) + (tiled ? 1 : 0)], k, threads);
       const int grid = (int)(units < sl ? units : sl);
       const int nu = (int)units;
       if constexpr (TILE_OK) {
@@ -322,8 +334,8 @@ hipError_t launch_cols_pipe(const PassArgs<T>& a, int mode, hipStream_t s) {
     if (ok) {
       constexpr int threads = (N / E) * W;
       constexpr int SPF = N / W;
-      static int slots = 0;  // same value on every thread that races to set it
-      if (slots == 0) slots = resident_groups((const void*)&k_cols_pipe<T, N, E, W, PS>, threads);
+      static std::atomic<int> slot{0};
+      const int slots = cached_slots(slot, (const void*)&k_cols_pipe<T, N, E, W, PS>, threads);
       int groups = slots / SPF;
       if (groups < 1) groups = 1;
       if (groups > a.batch) groups = a.batch;
@@ -351,8 +363,8 @@ hipError_t launch_small(const PassArgs<T>& a, int mode, hipStream_t s) {
     if constexpr (lds <= kLdsBudget) {
       if (mode == (MODE_FWD | MODE_INV) && a.evan == EV_PASS &&
           (a.flags & ~PIPE_OK_FLAGS) == 0) {
-        static int slots = 0;  // same value on every thread that races to set it
-        if (slots == 0) slots = resident_groups((const void*)&k_small_pipe<T, N, E, PS>, threads);
+        static std::atomic<int> slot{0};
+        const int slots = cached_slots(slot, (const void*)&k_small_pipe<T, N, E, PS>, threads);
         int groups = slots < a.batch ? slots : a.batch;
         if (groups < 1) groups = 1;
         int per = (a.batch + groups - 1) / groups;
