@@ -42,53 +42,65 @@ def parse():
     return p.parse_args()
 
 
-def cpu_baseline(n, budget_s):
-    """Time the oracle's field-propagation (numpy.fft, complex128, one thread: the
-    reference's own numpy fallback, beam2d.py:31-34, 217, 229) on a bounded sample."""
-    import oracle
-    rs = np.random.RandomState(0)
-    area, wl, z = 400e-4, 632e-7, 50e-4
-    u = (rs.standard_normal((n, n)) + 1j * rs.standard_normal((n, n))).astype(np.complex64)
-    oracle.propagate_field(u, area, wl, z)  # warm-up
-    t0 = time.perf_counter()
-    k = 0
-    while True:
-        oracle.propagate_field(u, area, wl, z)
-        k += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or k >= 200:
-            break
-    threads = os.environ.get("OMP_NUM_THREADS", "")
-    return {"value": k / el, "unit": "field-propagations/s", "cores": 1, "kind": "port",
-            "sample": f"{k} x {n}x{n} complex128 field-propagations (numpy.fft pocketfft, "
-                      f"single thread; grid + H rebuilt per call as the reference does) "
-                      f"in {el:.1f} s", "omp_num_threads": threads}
-
-
-def cpu_baseline_threaded(n, budget_s):
-    """The same field-propagation with scipy.fft on all host threads available to the
-    process (the reference's pyfftw ``threads=-1`` branch is not installed here)."""
-    import oracle.angular_spectrum as OA
+def _cpu_model():
     try:
-        workers = len(os.sched_getaffinity(0))
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:  # pragma: no cover
+        pass
+    return "unknown"
+
+
+def _host_threads():
+    try:
+        return len(os.sched_getaffinity(0))
     except AttributeError:  # pragma: no cover
-        workers = os.cpu_count() or 1
-    workers = max(1, min(workers, 16))
+        return os.cpu_count() or 1
+
+
+def _time_cpu(fn, budget_s, max_iter):
+    """Warm-up, then at least 5 timed calls until budget_s: (mean rate, best-of-5 rate,
+    calls, seconds)."""
+    fn()
+    times = []
+    t0 = time.perf_counter()
+    while len(times) < 5 or (time.perf_counter() - t0 < budget_s and len(times) < max_iter):
+        t1 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t1)
+    el = sum(times)
+    return len(times) / el, 1.0 / min(times[:5]), len(times), el
+
+
+def cpu_baseline(n, budget_s, workers=None):
+    """The oracle's restatement of Beam2D.propagate (numpy.fft on one thread: the
+    reference's own fallback, beam2d.py:31-34, 217, 229; or scipy.fft on `workers`
+    threads standing in for its pyfftw threads=-1 branch, beam2d.py:27-29) on a field
+    of a geometry built once, H rebuilt per call as the reference does.  The complex64
+    state (the GPU's arithmetic type) is `value`; complex128 (the reference default)
+    is reported beside it."""
+    import oracle.angular_spectrum as OA
     rs = np.random.RandomState(0)
     area, wl, z = 400e-4, 632e-7, 50e-4
     u = (rs.standard_normal((n, n)) + 1j * rs.standard_normal((n, n))).astype(np.complex64)
-    OA.propagate_field_threaded(u, area, wl, z, workers)
-    t0 = time.perf_counter()
-    k = 0
-    while True:
-        OA.propagate_field_threaded(u, area, wl, z, workers)
-        k += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or k >= 400:
-            break
-    return {"value": k / el, "unit": "field-propagations/s", "cores": workers, "kind": "port",
-            "sample": f"{k} x {n}x{n} complex128 field-propagations (scipy.fft, {workers} "
-                      f"workers; grid + H rebuilt per call) in {el:.1f} s"}
+    g = OA.construct_grids(area, n, wl)
+    res = {}
+    for name, dt in (("c64", np.complex64), ("c128", np.complex128)):
+        res[name] = _time_cpu(lambda: OA.propagate_field_prepared(u, g, z, dt, workers),
+                              budget_s / 2, 400)
+    mean64, best64, k64, el64 = res["c64"]
+    mean128, best128, k128, el128 = res["c128"]
+    lib = "numpy.fft pocketfft, one thread" if workers is None else f"scipy.fft, {workers} workers"
+    return {"value": mean64, "unit": "field-propagations/s",
+            "cores": 1 if workers is None else workers, "kind": "port",
+            "sample": f"{k64} x {n}x{n} complex64 field-propagations ({lib}; grids built "
+                      f"once, H rebuilt per call) in {el64:.1f} s; complex128: {k128} in "
+                      f"{el128:.1f} s",
+            "best_of_5": best64, "value_c128": mean128, "best_of_5_c128": best128,
+            "cpu_model": _cpu_model(), "host_threads": _host_threads(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS", "")}
 
 
 def profiled_traffic(kernel_name):
@@ -387,7 +399,8 @@ def main():
     cpu = cpu_mt = None
     if not args.no_cpu:
         cpu = cpu_baseline(n, args.cpu_seconds)
-        cpu_mt = cpu_baseline_threaded(n, min(args.cpu_seconds, 6.0))
+        cpu_mt = cpu_baseline(n, min(args.cpu_seconds, 6.0),
+                              workers=max(1, min(_host_threads(), 16)))
 
     line = {
         "metric": METRIC,

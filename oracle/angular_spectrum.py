@@ -353,6 +353,29 @@ def propagate_field(u, area_size, wl, z, phase_mask=None, aperture_radius=None,
     return ifft2(S)
 
 
+def propagate_field_prepared(u, g, z, dtype=np.complex128, workers=None):
+    """One ``Beam2D.propagate(z)`` on a field whose grids `g` (``construct_grids``) were
+    built once, in the state precision `dtype` (beam2d.py:111-114): spectrum =
+    fft2(field) (numpy >= 2 keeps complex64), H rebuilt from Kz in FP64 on every call
+    and multiplied in place, so a complex64 state casts the complex128 product back
+    (beam2d.py:484-489), then field = ifft2(spectrum) (:490).  `workers` runs scipy.fft
+    on that many threads instead of numpy.fft (the pyfftw ``threads=-1`` stand-in,
+    beam2d.py:27-29).  This is the per-field CPU cost of a reference user who reuses one
+    Beam2D geometry for many fields."""
+    u = np.asarray(u, dtype=dtype)
+    if workers is None:
+        S = np.fft.fft2(u)
+    else:
+        import scipy.fft as sfft
+        S = sfft.fft2(u, workers=workers)
+    if z != 0:
+        S *= transfer_function(g.Kz, z)
+    if workers is None:
+        return np.fft.ifft2(S)
+    import scipy.fft as sfft
+    return sfft.ifft2(S, workers=workers)
+
+
 def propagate_field_threaded(u, area_size, wl, z, workers):
     """``propagate_field`` (no mask, no lens) with scipy.fft on `workers` threads: the
     stand-in for the reference's pyfftw branch with ``threads=-1`` (beam2d.py:27-29,

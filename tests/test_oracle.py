@@ -370,3 +370,22 @@ def test_modes_round_trip_and_padding():
     # a unit coefficient reconstructs its own padded mode
     b.construct_by_modes(list(cm), np.eye(m)[5])
     np.testing.assert_allclose(b.field, np.pad(cm[5].reshape(nb, nb), 8), atol=1e-15)
+
+
+def test_prepared_grid_propagation_in_both_state_precisions():
+    """propagate_field_prepared (grids built once, the CPU baseline's form) agrees with
+    propagate_field: to rounding in complex128 (numpy and scipy.fft), to complex64
+    rounding in a complex64 state, whose spectrum and field stay complex64."""
+    from oracle.angular_spectrum import construct_grids, propagate_field, propagate_field_prepared
+    rs = np.random.RandomState(3)
+    n, area, wl, z = 128, 300e-4, 632e-7, 0.02
+    u = (rs.standard_normal((n, n)) + 1j * rs.standard_normal((n, n))).astype(np.complex64)
+    g = construct_grids(area, n, wl)
+    ref = propagate_field(u, area, wl, z)
+    a = propagate_field_prepared(u, g, z, np.complex128)
+    b = propagate_field_prepared(u, g, z, np.complex128, workers=2)
+    c = propagate_field_prepared(u, g, z, np.complex64)
+    assert a.dtype == np.complex128 and c.dtype == np.complex64
+    assert np.linalg.norm(a - ref) / np.linalg.norm(ref) < 1e-14
+    assert np.linalg.norm(b - ref) / np.linalg.norm(ref) < 1e-14
+    assert 1e-9 < np.linalg.norm(c - ref) / np.linalg.norm(ref) < 1e-6
