@@ -131,16 +131,21 @@ __device__ __forceinline__ T* uniform_ptr(T* p) {
 // the hint changes how the next sweep finds the plane, so it is chosen per access
 // site by measurement.  LP2D_NT is a mask of the sites that carry it.  Measured at
 // 32 x 1024^2 complex64: the row-pass loads gain 5-7 us per row pass and 8 us in the
-// following column pass, the gradient pass 28 us; hinted stores and column loads
-// were slower in every combination, the whole-field kernels indifferent.
+// following column pass, the gradient pass 28 us; the inverse row pass's stores (the
+// final field) leave the forward step unchanged and speed up the adjoint's column
+// pass (130 -> 124 us), the speckle chain (5 %) and the uneven batch (1.5 %).  Hinted
+// forward-row and column stores, and hinted column loads, were slower; the
+// whole-field kernels are indifferent.
 enum : int {
-  NT_ROW_LD = 1, NT_ROW_ST = 2,    // row passes
+  NT_ROW_LD = 1,                   // row passes: loads
+  NT_ROW_ST = 2,                   //             stores of the forward pass
   NT_COL_LD = 4, NT_COL_ST = 8,    // column passes
   NT_GRAD = 16,                    // adjoint gradient row pass
   NT_SMALL = 32,                   // whole-field kernels
+  NT_ROWI_ST = 64,                 // row passes: stores of the inverse pass
 };
 #ifndef LP2D_NT
-#define LP2D_NT (NT_ROW_LD | NT_GRAD)
+#define LP2D_NT (NT_ROW_LD | NT_GRAD | NT_ROWI_ST)
 #endif
 template <int SITE, typename T>
 __device__ __forceinline__ cx<T> gld(const LP2D_GLB cx<T>* p) {
@@ -532,11 +537,11 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
     if ((a.flags & F_NO_STORE) == 0) {
       if (!INV && tiled) {
 #pragma unroll
-        for (int m = 0; m < E; ++m) gst<NT_ROW_ST>(dst + (lo_t + (unsigned)(kTileRows * TL * m)), v[m]);
+        for (int m = 0; m < E; ++m) gst<INV ? NT_ROWI_ST : NT_ROW_ST>(dst + (lo_t + (unsigned)(kTileRows * TL * m)), v[m]);
       } else {
 #pragma unroll
         for (int m = 0; m < E; ++m)
-          if (valid) gst<NT_ROW_ST>(dst + (lo + (unsigned)(TL * m)), v[m]);
+          if (valid) gst<INV ? NT_ROWI_ST : NT_ROW_ST>(dst + (lo + (unsigned)(TL * m)), v[m]);
       }
     }
   } else {
@@ -544,11 +549,11 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
     if (!INV && tiled) {
 #pragma unroll
       for (int m = 0; m < E; ++m)
-        gst<NT_ROW_ST>(dst + (lo_t + (unsigned)(kTileRows * TL * m)), mk(v[m].re * sc, v[m].im * sc));
+        gst<INV ? NT_ROWI_ST : NT_ROW_ST>(dst + (lo_t + (unsigned)(kTileRows * TL * m)), mk(v[m].re * sc, v[m].im * sc));
     } else {
 #pragma unroll
       for (int m = 0; m < E; ++m)
-        if (valid) gst<NT_ROW_ST>(dst + (lo + (unsigned)(TL * m)), mk(v[m].re * sc, v[m].im * sc));
+        if (valid) gst<INV ? NT_ROWI_ST : NT_ROW_ST>(dst + (lo + (unsigned)(TL * m)), mk(v[m].re * sc, v[m].im * sc));
     }
   }
 }
@@ -621,7 +626,8 @@ __device__ __forceinline__ void rows_pipe_body(const PassArgs<T>& a, int u0, int
     }
     LP2D_GLB cx<T>* __restrict__ dst = glb(uniform_ptr(a.dst + tile_off(u)));
 #pragma unroll
-    for (int m = 0; m < E; ++m) gst<NT_ROW_ST>(dst + (st_o + ST_S * (unsigned)m), cur[m]);
+    for (int m = 0; m < E; ++m)
+      gst<INV ? NT_ROWI_ST : NT_ROW_ST>(dst + (st_o + ST_S * (unsigned)m), cur[m]);
   };
   if (u0 >= nunits) return;
   fetch(u0);
