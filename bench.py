@@ -196,6 +196,9 @@ def extra_legs(torch, P, L, lib, dev):
     return out
 
 
+MFMA_FP64_PEAK_TFS = 78.6  # MI355X dense FP64 matrix peak
+
+
 def modes_leg(torch, P, dev):
     """Mode-basis projection and reconstruction: 64 modes of 256^2 points against 32
     complex64 fields of 512^2 (central 256^2 window).  Algorithmic bytes: the mode
@@ -223,6 +226,13 @@ def modes_leg(torch, P, dev):
         el = (time.perf_counter() - t0) / 20
         res[name] = {"us_per_call": round(el * 1e6, 1), "GBs": round(alg / el / 1e9, 1),
                      "hbm_frac": round(alg / el / 1e9 / HBM_PEAK_GBS, 4)}
+    # the batch projection runs on the FP64 matrix cores: 4 real products of
+    # (M x P) by (P x B) per call, 2 flops per multiply-add
+    fl = 8.0 * M * B * nb * nb
+    tfs = fl / (res["project"]["us_per_call"] * 1e-6) / 1e12
+    res["project"].update({"path": "FP64 MFMA (v_mfma_f64_16x16x4f64)",
+                           "mfma_TFLOPs": round(tfs, 2),
+                           "mfma_frac": round(tfs / MFMA_FP64_PEAK_TFS, 4)})
     mh = modes[:, :].cpu().numpy().astype(np.complex128)
     fh = f[0, 128:384, 128:384].reshape(-1).cpu().numpy().astype(np.complex128)
     t0 = time.perf_counter()

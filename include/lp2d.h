@@ -175,6 +175,17 @@ int lp2d_coarse(int n, int mean_order, int batch, int dtype, const void* in, voi
  * in `dtype`; coeffs is (batch, nmodes) complex128 (FP64 accumulation), overwritten. */
 int lp2d_modes_project(int n, int nb, int nmodes, int batch, int dtype, const void* modes,
                        const void* field, void* coeffs, void* stream);
+/* The same projection on the matrix cores (FP64 MFMA, every product and sum in FP64)
+ * for batches of fields: each workgroup reduces a pixel range of a 64-mode x 32-field
+ * tile into `work`, and a second kernel sums the ranges into coeffs.  work_bytes must
+ * be at least lp2d_modes_project_ws_size(n, nb, nmodes, batch); work must not alias the
+ * other arrays, and its contents are undefined on return.  work == NULL runs
+ * lp2d_modes_project.  Same reference interface as lp2d_modes_project
+ * (beam2d.py:563-603). */
+long long lp2d_modes_project_ws_size(int n, int nb, int nmodes, int batch);
+int lp2d_modes_project_ws(int n, int nb, int nmodes, int batch, int dtype, const void* modes,
+                          const void* field, void* coeffs, void* work, long long work_bytes,
+                          void* stream);
 /* Reconstruction (Beam2D.construct_by_modes, beam2d.py:653-667): field[b] =
  * sum_k coeffs[b, k] * mode_k zero-padded into the centre of the n x n grid.  coeffs
  * (batch, nmodes) complex128; field (batch, n, n) in `dtype`, overwritten. */

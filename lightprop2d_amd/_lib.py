@@ -42,6 +42,7 @@ EXPORTED = [
     "lp2d_set_variant", "lp2d_get_variant", "lp2d_variant_name", "lp2d_trace_enable",
     "lp2d_trace_collect", "lp2d_moments", "lp2d_set_chunk", "lp2d_get_chunk",
     "lp2d_profile", "lp2d_expand", "lp2d_coarse", "lp2d_modes_project",
+    "lp2d_modes_project_ws_size", "lp2d_modes_project_ws",
     "lp2d_modes_construct",
 ]
 
@@ -129,6 +130,8 @@ _SIGS = {
     "lp2d_expand": (_I, [_I, _I, _I, _I, _P, _P, _P]),
     "lp2d_coarse": (_I, [_I, _I, _I, _I, _P, _P, _P]),
     "lp2d_modes_project": (_I, [_I, _I, _I, _I, _I, _P, _P, _P, _P]),
+    "lp2d_modes_project_ws_size": (C.c_longlong, [_I, _I, _I, _I]),
+    "lp2d_modes_project_ws": (_I, [_I, _I, _I, _I, _I, _P, _P, _P, _P, C.c_longlong, _P]),
     "lp2d_modes_construct": (_I, [_I, _I, _I, _I, _I, _P, _P, _P, _P]),
 }
 

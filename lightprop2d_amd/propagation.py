@@ -341,19 +341,37 @@ def mode_matrix(modes, dtype, device) -> torch.Tensor:
     return m.to(device=device, dtype=dtype).contiguous()
 
 
-def modes_project(field: torch.Tensor, modes: torch.Tensor) -> torch.Tensor:
+# Batches from this size on project on the matrix cores (lp2d_modes_project_ws).
+MFMA_MIN_BATCH = 8
+
+
+def modes_project(field: torch.Tensor, modes: torch.Tensor,
+                  matrix_cores: Optional[bool] = None) -> torch.Tensor:
     """coeffs[b, k] = sum_p modes[k, p] * field[b, window(p)] (unconjugated, as the
     reference's ``modes_matrix @ flatten_field``, beam2d.py:597); complex128 result,
-    shape (nmodes,) for one field or (batch, nmodes)."""
+    shape (nmodes,) for one field or (batch, nmodes).
+
+    Batches of MFMA_MIN_BATCH fields or more run as a complex GEMM on the FP64 matrix
+    cores with a workspace allocated here; matrix_cores=False / True forces the vector
+    kernel / the matrix-core one."""
     lib = _lib()
     n, batch, dt = _field_geom(field)
     modes = mode_matrix(modes, field.dtype, field.device)
     nm = modes.shape[0]
     nb = int(round(modes.shape[1] ** 0.5))
     out = torch.empty((batch, nm), dtype=torch.complex128, device=field.device)
+    if matrix_cores is None:
+        matrix_cores = batch >= MFMA_MIN_BATCH
     with torch.cuda.device(field.device):
-        L.check(lib.lp2d_modes_project(n, nb, nm, batch, dt, _ptr(modes), _ptr(field), _ptr(out),
-                                       _stream(field)), "modes_project")
+        if matrix_cores:
+            wb = int(lib.lp2d_modes_project_ws_size(n, nb, nm, batch))
+            work = torch.empty(max(wb, 16), dtype=torch.uint8, device=field.device)
+            L.check(lib.lp2d_modes_project_ws(n, nb, nm, batch, dt, _ptr(modes), _ptr(field),
+                                              _ptr(out), _ptr(work), wb, _stream(field)),
+                    "modes_project")
+        else:
+            L.check(lib.lp2d_modes_project(n, nb, nm, batch, dt, _ptr(modes), _ptr(field),
+                                           _ptr(out), _stream(field)), "modes_project")
     return out[0] if field.dim() == 2 else out
 
 

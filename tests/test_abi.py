@@ -117,5 +117,16 @@ def test_argument_errors_of_the_generator_resample_and_mode_entry_points(lib):
     assert lib.lp2d_coarse(64, 2, 1, 0, None, p, None) == _lib.LP2D_E_ARG
     assert lib.lp2d_modes_project(64, 65, 3, 1, 0, p, p, p, None) == _lib.LP2D_E_SIZE
     assert lib.lp2d_modes_project(64, 63, 3, 1, 0, p, p, p, None) == _lib.LP2D_E_SIZE
+    # matrix-core projection: workspace size and aliasing are checked before launch
+    need = lib.lp2d_modes_project_ws_size(512, 256, 64, 32)
+    assert need > 0 and need % 16 == 0
+    assert lib.lp2d_modes_project_ws_size(64, 65, 3, 1) == 0
+    w = C.c_void_p(64)
+    assert lib.lp2d_modes_project_ws(512, 256, 64, 32, 0, p, C.c_void_p(32), C.c_void_p(48), w,
+                                     need - 16, None) == _lib.LP2D_E_ARG
+    assert b"workspace too small" in lib.lp2d_last_error()
+    assert lib.lp2d_modes_project_ws(512, 256, 64, 32, 0, p, C.c_void_p(32), w, w, need,
+                                     None) == _lib.LP2D_E_ARG
+    assert lib.lp2d_modes_project_ws(64, 63, 3, 1, 0, p, p, p, w, 1 << 20, None) == _lib.LP2D_E_SIZE
     assert lib.lp2d_modes_construct(64, 32, 0, 1, 0, p, p, p, None) == _lib.LP2D_E_SIZE
     assert lib.lp2d_modes_construct(64, 32, 3, 1, 2, p, p, p, None) == _lib.LP2D_E_DTYPE

@@ -88,3 +88,27 @@ def test_modes_reject_bad_windows():
         P.modes_project(f, torch.zeros((3, 63 * 63), dtype=torch.complex64, device=DEV))
     with pytest.raises(Lp2dError):
         P.modes_project(f, torch.zeros((3, 10), dtype=torch.complex64, device=DEV))
+
+
+@pytest.mark.parametrize("dt,tol", [(np.complex64, 1e-6), (np.complex128, 1e-13)],
+                         ids=["c64", "c128"])
+@pytest.mark.parametrize("n,nb,m,batch", [(512, 256, 64, 32), (256, 198, 70, 33),
+                                          (128, 100, 17, 8), (64, 64, 130, 40), (64, 2, 3, 1)])
+def test_project_on_matrix_cores(dt, tol, n, nb, m, batch):
+    """The FP64-MFMA projection (64-mode x 32-field tiles, 32-pixel chunks, per-range
+    partial sums) against numpy and against the vector kernel: ragged mode, field and
+    pixel counts (70 modes, 33 fields, 198^2 pixels) leave partial tiles and chunks;
+    both accumulate in FP64, so they agree to summation-order rounding."""
+    from lightprop2d_amd import propagation as P
+    rs = np.random.RandomState(n + m + batch)
+    modes = cplx(rs, (m, nb * nb)).astype(dt)
+    f = cplx(rs, (batch, n, n)).astype(dt)
+    d = (n - nb) // 2
+    win = f[:, d:n - d, d:n - d].reshape(batch, -1).astype(np.complex128)
+    exp = win @ modes.astype(np.complex128).T
+    ft, mt = torch.from_numpy(f).to(DEV), torch.from_numpy(modes).to(DEV)
+    print(f"n={n} nb={nb} modes={m} batch={batch} {dt.__name__}", flush=True)
+    got = P.modes_project(ft, mt, matrix_cores=True).reshape(batch, m)
+    vec = P.modes_project(ft, mt, matrix_cores=False).reshape(batch, m)
+    assert rel_l2(got.cpu().numpy(), exp) < tol
+    assert rel_l2(got.cpu().numpy(), vec.cpu().numpy()) < 1e-13
