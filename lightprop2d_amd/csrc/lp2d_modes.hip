@@ -124,46 +124,76 @@ __global__ void k_zero(double2* p, long long count) {
 
 // field[b, j, i] = sum_k coeffs[b, k] * modes[k, window^-1(j, i)] inside the window,
 // 0 outside (beam2d.py:664-666 with the zero-padded basis of :555-560).  Accumulated
-// in the field's precision, modes in order k = 0, 1, ... as numpy's einsum; each
-// thread serves one pixel of up to kFieldsPerPass fields, reading each mode element
-// once for all of them.  Grid: x = rows x groups of fields, y = 256-column chunks.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_construct(const C2<T>* __restrict__ modes,
-                                                        const double2* __restrict__ coeffs,
-                                                        C2<T>* __restrict__ field, int n, int nb,
-                                                        int nmodes, int batch) {
+// in the field's precision, modes in order k = 0, 1, ... as numpy's einsum.  A
+// workgroup serves PX pixels of one row for FG x KF fields: wave-sized thread groups
+// take KF fields each for the same pixels, so the waves of a workgroup read the same
+// mode elements (one HBM read, L1 hits after it) and a batch of FG x KF fields reads
+// the mode matrix once.  The coefficients of a block of kConstructModes modes are
+// staged in LDS in the field's precision (broadcast reads in the inner loop).  Grid:
+// x = rows x groups of FG x KF fields, y = PX-column chunks; a row outside the window
+// only writes zeros.
+constexpr int kConstructModes = 64;
+template <typename T, int KF, int FG, int PX>
+__global__ __launch_bounds__(PX * FG) void k_construct(const C2<T>* __restrict__ modes,
+                                                       const double2* __restrict__ coeffs,
+                                                       C2<T>* __restrict__ field, int n, int nb,
+                                                       int nmodes, int batch) {
+  constexpr int NF = KF * FG;  // fields per workgroup
+  constexpr int THREADS = PX * FG;
+  __shared__ C2<T> cs[kConstructModes][NF];
   const int row = blockIdx.x;
   const int g = row / n, j = row - g * n;
-  const int b0 = g * kFieldsPerPass;
-  const int nf = batch - b0 < kFieldsPerPass ? batch - b0 : kFieldsPerPass;
-  const int i = blockIdx.y * kThreads + threadIdx.x;
-  if (i >= n) return;
+  const int b0 = g * NF;
+  const int nf = batch - b0 < NF ? batch - b0 : NF;
+  const int fg = threadIdx.x / PX;           // this thread's group of KF fields
+  const int i = blockIdx.y * PX + (threadIdx.x - fg * PX);
   const int dN = (n - nb) / 2;
   const int jj = j - dN, ii = i - dN;
-  T sr[kFieldsPerPass] = {}, si[kFieldsPerPass] = {};
-  if (jj >= 0 && jj < nb && ii >= 0 && ii < nb) {
-    const long long p = (long long)jj * nb + ii;
+  const bool in_row = jj >= 0 && jj < nb;  // the same for the whole workgroup
+  const bool in_px = in_row && i < n && ii >= 0 && ii < nb;
+  T sr[KF] = {}, si[KF] = {};
+  if (in_row) {
     const long long npix = (long long)nb * nb;
-    for (int k = 0; k < nmodes; ++k) {
-      const C2<T> m = modes[(long long)k * npix + p];
+    const long long p = (long long)jj * nb + (in_px ? ii : 0);
+    for (int k0 = 0; k0 < nmodes; k0 += kConstructModes) {
+      const int nk = nmodes - k0 < kConstructModes ? nmodes - k0 : kConstructModes;
+      __syncthreads();  // the previous block's coefficients are no longer read
+      for (int e = threadIdx.x; e < kConstructModes * NF; e += THREADS) {
+        const int kk = e / NF, f = e - (e / NF) * NF;
+        C2<T> c;
+        c.x = 0;
+        c.y = 0;
+        if (kk < nk && f < nf) {
+          const double2 d = coeffs[(long long)(b0 + f) * nmodes + k0 + kk];
+          c.x = (T)d.x;
+          c.y = (T)d.y;
+        }
+        cs[kk][f] = c;
+      }
+      __syncthreads();
+      if (in_px) {
+        for (int kk = 0; kk < nk; ++kk) {
+          const C2<T> m = modes[(long long)(k0 + kk) * npix + p];
 #pragma unroll
-      for (int f = 0; f < kFieldsPerPass; ++f) {
-        if (f < nf) {
-          const double2 c = coeffs[(long long)(b0 + f) * nmodes + k];
-          const T cr = (T)c.x, ci = (T)c.y;
-          sr[f] += m.x * cr - m.y * ci;
-          si[f] += m.x * ci + m.y * cr;
+          for (int f = 0; f < KF; ++f) {
+            const C2<T> c = cs[kk][fg * KF + f];
+            sr[f] += m.x * c.x - m.y * c.y;
+            si[f] += m.x * c.y + m.y * c.x;
+          }
         }
       }
     }
   }
+  if (i < n) {
 #pragma unroll
-  for (int f = 0; f < kFieldsPerPass; ++f) {
-    if (f < nf) {
-      C2<T> o;
-      o.x = sr[f];
-      o.y = si[f];
-      field[((long long)(b0 + f) * n + j) * n + i] = o;
+    for (int f = 0; f < KF; ++f) {
+      const int fl = fg * KF + f;
+      if (fl < nf) {
+        C2<T> o;
+        o.x = sr[f];
+        o.y = si[f];
+        field[((long long)(b0 + fl) * n + j) * n + i] = o;
+      }
     }
   }
 }
@@ -450,15 +480,31 @@ extern "C" int lp2d_modes_construct(int n, int nb, int nmodes, int batch, int dt
                                     void* stream) {
   if (int rc = check(n, nb, nmodes, batch, dtype, modes, coeffs, field)) return rc;
   hipStream_t s = (hipStream_t)stream;
-  const int groups = (batch + kFieldsPerPass - 1) / kFieldsPerPass;
-  const dim3 grid((unsigned)(groups * n), (unsigned)((n + kThreads - 1) / kThreads));
-  if (dtype == LP2D_C64)
-    k_construct<float><<<grid, kThreads, 0, s>>>((const C2<float>*)modes, (const double2*)coeffs,
-                                                  (C2<float>*)field, n, nb, nmodes, batch);
-  else
-    k_construct<double><<<grid, kThreads, 0, s>>>((const C2<double>*)modes,
-                                                   (const double2*)coeffs, (C2<double>*)field, n,
-                                                   nb, nmodes, batch);
+  // batches of more than 8 fields: 4 groups of 8 per workgroup (64 pixels), the mode
+  // matrix read once per 32 fields; otherwise 8 fields for 256 pixels
+  const bool wide = batch > kFieldsPerPass;
+  const int nf = wide ? 4 * kFieldsPerPass : kFieldsPerPass;
+  const int px = wide ? 64 : kThreads;
+  const long long rows = (long long)((batch + nf - 1) / nf) * n;
+  if (rows > 0x7fffffffLL) return lp2d::set_error(LP2D_E_SIZE, "batch x n too large");
+  const dim3 grid((unsigned)rows, (unsigned)((n + px - 1) / px));
+  if (dtype == LP2D_C64) {
+    if (wide)
+      k_construct<float, kFieldsPerPass, 4, 64><<<grid, 256, 0, s>>>(
+          (const C2<float>*)modes, (const double2*)coeffs, (C2<float>*)field, n, nb, nmodes, batch);
+    else
+      k_construct<float, kFieldsPerPass, 1, kThreads><<<grid, kThreads, 0, s>>>(
+          (const C2<float>*)modes, (const double2*)coeffs, (C2<float>*)field, n, nb, nmodes, batch);
+  } else {
+    if (wide)
+      k_construct<double, kFieldsPerPass, 4, 64><<<grid, 256, 0, s>>>(
+          (const C2<double>*)modes, (const double2*)coeffs, (C2<double>*)field, n, nb, nmodes,
+          batch);
+    else
+      k_construct<double, kFieldsPerPass, 1, kThreads><<<grid, kThreads, 0, s>>>(
+          (const C2<double>*)modes, (const double2*)coeffs, (C2<double>*)field, n, nb, nmodes,
+          batch);
+  }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? LP2D_OK : lp2d::set_error(LP2D_E_HIP, hipGetErrorString(e));
 }

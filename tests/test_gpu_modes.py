@@ -95,7 +95,7 @@ def test_modes_reject_bad_windows():
 @pytest.mark.parametrize("n,nb,m,batch", [(512, 256, 64, 32), (256, 198, 70, 33),
                                           (128, 100, 17, 8), (64, 64, 130, 40), (64, 2, 3, 1)])
 def test_project_on_matrix_cores(dt, tol, n, nb, m, batch):
-    """The FP64-MFMA projection (64-mode x 32-field tiles, 32-pixel chunks, per-range
+    """Batches through the wide kernels.  The FP64-MFMA projection (64-mode x 32-field tiles, 32-pixel chunks, per-range
     partial sums) against numpy and against the vector kernel: ragged mode, field and
     pixel counts (70 modes, 33 fields, 198^2 pixels) leave partial tiles and chunks;
     both accumulate in FP64, so they agree to summation-order rounding."""
@@ -112,3 +112,10 @@ def test_project_on_matrix_cores(dt, tol, n, nb, m, batch):
     vec = P.modes_project(ft, mt, matrix_cores=False).reshape(batch, m)
     assert rel_l2(got.cpu().numpy(), exp) < tol
     assert rel_l2(got.cpu().numpy(), vec.cpu().numpy()) < 1e-13
+    # reconstruction of the same batch (32 fields per pass above 8, coefficients of
+    # 64-mode blocks staged in LDS: 70 and 130 modes leave partial blocks)
+    c = cplx(rs, (batch, m))
+    fe = np.einsum('ijk,bi->bjk', np.pad(modes.astype(np.complex128).reshape(m, nb, nb),
+                                          ((0, 0), (d, d), (d, d))), c)
+    fg = P.modes_construct(mt, torch.from_numpy(c), n, dtype=ft.dtype, device=torch.device(DEV))
+    assert rel_l2(fg.cpu().numpy().reshape(batch, n, n), fe) < tol
