@@ -3,6 +3,7 @@
 // bound by HBM: expand reads n^2 and writes n_out^2 elements, coarse reads n^2 and
 // writes (n/m)^2.
 #include <hip/hip_runtime.h>
+#include <stdint.h>
 
 #include "../../include/lp2d.h"
 
@@ -55,6 +56,58 @@ __global__ void k_coarse(const C* __restrict__ in, C* __restrict__ out, int n, i
   out[(long long)row * no + I] = o;
 }
 
+// complex64 fast paths: whole 16-byte vectors per lane.  8 fields 1024^2 <-> 2048^2:
+// expand 86.7 -> 83.5 us, coarse 70 -> 49 us (with nontemporal output stores, which
+// help the coarse mean and not the expansion).
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Expand with four output elements (i0 .. i0+3, n_out a multiple of 4) per thread,
+// written as two 16-byte stores.
+__global__ void k_expand4(const float2* __restrict__ in, float2* __restrict__ out, int n,
+                          int n_out, int d) {
+  const int row = blockIdx.x;
+  const int b = row / n_out, j = row - b * n_out;
+  const int i0 = 4 * (blockIdx.y * blockDim.x + threadIdx.x);
+  if (i0 >= n_out) return;
+  int js = j - d;
+  js = js < 0 ? 0 : (js >= n ? n - 1 : js);
+  const float2* src = in + ((long long)b * n + js) * n;
+  float2 v[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    int is = i0 + q - d;
+    is = is < 0 ? 0 : (is >= n ? n - 1 : is);
+    v[q] = src[is];
+  }
+  v4f* dst = (v4f*)(out + (long long)row * n_out + i0);
+  v4f a, c;
+  a.x = v[0].x; a.y = v[0].y; a.z = v[1].x; a.w = v[1].y;
+  c.x = v[2].x; c.y = v[2].y; c.z = v[3].x; c.w = v[3].y;
+  // plain stores: with the nontemporal hint this kernel measured 119.5 us against
+  // 83.5 us (8 x 1024^2 -> 2048^2)
+  dst[0] = a;
+  dst[1] = c;
+}
+
+// 2 x 2 block mean with two outputs (I0, I0 + 1; no a multiple of 2) per thread: each
+// block row is one 16-byte load; sums in the order of k_coarse (row by row).
+__global__ void k_coarse2(const float2* __restrict__ in, float2* __restrict__ out, int n) {
+  const int no = n / 2;
+  const int row = blockIdx.x;
+  const int b = row / no, J = row - b * no;
+  const int I0 = 2 * (blockIdx.y * blockDim.x + threadIdx.x);
+  if (I0 >= no) return;
+  const v4f* r0 = (const v4f*)(in + ((long long)b * n + 2LL * J) * n + 2LL * I0);
+  const v4f* r1 = (const v4f*)((const float2*)r0 + n);
+  const v4f a0 = r0[0], a1 = r0[1], c0 = r1[0], c1 = r1[1];
+  v4f o;
+  o.x = (((a0.x + a0.z) + c0.x) + c0.z) / 4.0f;
+  o.y = (((a0.y + a0.w) + c0.y) + c0.w) / 4.0f;
+  o.z = (((a1.x + a1.z) + c1.x) + c1.z) / 4.0f;
+  o.w = (((a1.y + a1.w) + c1.y) + c1.w) / 4.0f;
+  __builtin_nontemporal_store(o, (v4f*)(out + (long long)row * no + I0));
+}
+
 int prep() {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return lp2d::set_error(LP2D_E_HIP, "hipGetDevice failed");
@@ -76,7 +129,10 @@ extern "C" int lp2d_expand(int n, int n_out, int batch, int dtype, const void* i
   const int d = (n_out - n) / 2;
   hipStream_t s = (hipStream_t)stream;
   const dim3 grid((unsigned)(batch * n_out), (unsigned)((n_out + 255) / 256));
-  if (dtype == LP2D_C64)
+  if (dtype == LP2D_C64 && n_out % 4 == 0 && ((uintptr_t)out & 15) == 0) {
+    const dim3 g4((unsigned)(batch * n_out), (unsigned)((n_out / 4 + 255) / 256));
+    k_expand4<<<g4, 256, 0, s>>>((const float2*)in, (float2*)out, n, n_out, d);
+  } else if (dtype == LP2D_C64)
     k_expand<float2><<<grid, 256, 0, s>>>((const float2*)in, (float2*)out, n, n_out, d);
   else
     k_expand<double2><<<grid, 256, 0, s>>>((const double2*)in, (double2*)out, n, n_out, d);
@@ -96,7 +152,11 @@ extern "C" int lp2d_coarse(int n, int mean_order, int batch, int dtype, const vo
   if ((long long)batch * no > 0x7fffffffLL) return set_error(LP2D_E_SIZE, "batch too large");
   hipStream_t s = (hipStream_t)stream;
   const dim3 grid((unsigned)(batch * no), (unsigned)((no + 255) / 256));
-  if (dtype == LP2D_C64)
+  if (dtype == LP2D_C64 && mean_order == 2 && no % 2 == 0 && ((uintptr_t)in & 15) == 0 &&
+      ((uintptr_t)out & 15) == 0) {
+    const dim3 g2((unsigned)(batch * no), (unsigned)((no / 2 + 255) / 256));
+    k_coarse2<<<g2, 256, 0, s>>>((const float2*)in, (float2*)out, n);
+  } else if (dtype == LP2D_C64)
     k_coarse<float, float2><<<grid, 256, 0, s>>>((const float2*)in, (float2*)out, n, mean_order);
   else
     k_coarse<double, double2><<<grid, 256, 0, s>>>((const double2*)in, (double2*)out, n,

@@ -68,3 +68,18 @@ def test_beam2d_expand_then_coarse_vs_oracle(bits):
         assert rel_l2(b.field.cpu().numpy(), o.field) < tol, op
         assert rel_l2(b.spectrum.cpu().numpy(), o.spectrum) < tol, op
     assert tuple(b.X.shape) == (n,) and tuple(b.Kz.shape) == (n, n)
+
+
+@pytest.mark.parametrize("n", [64, 1024])
+def test_coarse_vector_and_generic_paths_agree_bitwise(n):
+    """The complex64 2 x 2 coarse with 16-byte row loads against the one-element kernel,
+    which runs for an input that is not 16-byte aligned: same values, same summation
+    order, so bit for bit.  (The 4-element expand is checked bit-exact against numpy
+    above; widths that are not a multiple of 4 take the one-element kernel.)"""
+    from lightprop2d_amd import propagation as P
+    rs = np.random.RandomState(n)
+    u = torch.from_numpy(cplx(rs, (2, n, n), np.complex64)).to(DEV)
+    flat = torch.empty(2 * n * n + 1, dtype=torch.complex64, device=DEV)
+    um = flat[1:].view(2, n, n)  # 8-byte offset: the generic kernel
+    um.copy_(u)
+    assert torch.equal(P.coarse(u, 2), P.coarse(um, 2))
