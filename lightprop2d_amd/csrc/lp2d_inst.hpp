@@ -92,7 +92,7 @@ __global__ __launch_bounds__((N / E) * W) LP2D_OCC void k_cols(PassArgs<T> a, in
 // Pipelined column pass: grid (N/W strips) x (field groups of `per` <= kPipeMaxPer
 // fields); the twiddle tables of the strip's passes and the group's field pointers
 // and optics are staged in LDS once per workgroup.
-template <typename T, int N, int E, int W, int PS>
+template <typename T, int N, int E, int W, int PS, int HOISTH = HOIST_NONE>
 __global__ __launch_bounds__((N / E) * W) LP2D_OCC void k_cols_pipe(PassArgs<T> a, int per) {
   __shared__ cx<T> lds[W * PaddedLen<N, PS>::value];
   __shared__ cx<T> twl[TwlLen<N, E>::value];
@@ -109,7 +109,7 @@ __global__ __launch_bounds__((N / E) * W) LP2D_OCC void k_cols_pipe(PassArgs<T> 
   twl_fill<T, N, E, THREADS>(twl, a.tw, threadIdx.x);
   pipe_stage_fields(a, b0, b1, pf, threadIdx.x);
   __syncthreads();
-  cols_pipe_body<T, N, E, W, PS>(a, b0, b1, x0, lds, twl, pf, threadIdx.x);
+  cols_pipe_body<T, N, E, W, PS, HOISTH>(a, b0, b1, x0, lds, twl, pf, threadIdx.x);
 }
 
 // Whole field in LDS, pipelined over groups of `per` <= kPipeMaxPer fields.
@@ -323,7 +323,7 @@ hipError_t launch_cols(const PassArgs<T>& a, int mode, hipStream_t s) {
 // multipliers, other evanescent modes, single-direction passes) runs the plain class
 // of the same shape.  Each workgroup takes `per` fields so that the grid is about one
 // round of resident workgroups.
-template <typename T, int N, int E, int W, int PS>
+template <typename T, int N, int E, int W, int PS, int HOISTH = HOIST_NONE>
 hipError_t launch_cols_pipe(const PassArgs<T>& a, int mode, hipStream_t s) {
   const bool ok = pipe_eligible(a, mode);
   // the block-tiled layout is only defined for strips of kTileCols columns, and only
@@ -334,14 +334,28 @@ hipError_t launch_cols_pipe(const PassArgs<T>& a, int mode, hipStream_t s) {
     if (ok) {
       constexpr int threads = (N / E) * W;
       constexpr int SPF = N / W;
-      static std::atomic<int> slot{0};
-      const int slots = cached_slots(slot, (const void*)&k_cols_pipe<T, N, E, W, PS>, threads);
+      // the hoisted factor needs one set of optics for the whole call
+      const bool hoist = HOISTH != HOIST_NONE && a.fopt == nullptr;
+      static std::atomic<int> slot[2];
+      int slots = 0;
+      if constexpr (HOISTH != HOIST_NONE) {
+        if (hoist)
+          slots = cached_slots(slot[1], (const void*)&k_cols_pipe<T, N, E, W, PS, HOISTH>, threads);
+      }
+      if (!hoist)
+        slots = cached_slots(slot[0], (const void*)&k_cols_pipe<T, N, E, W, PS>, threads);
       int groups = slots / SPF;
       if (groups < 1) groups = 1;
       if (groups > a.batch) groups = a.batch;
       int per = (a.batch + groups - 1) / groups;
       if (per > kPipeMaxPer) per = kPipeMaxPer;
       groups = (a.batch + per - 1) / per;
+      if constexpr (HOISTH != HOIST_NONE) {
+        if (hoist) {
+          k_cols_pipe<T, N, E, W, PS, HOISTH><<<(unsigned)(groups * SPF), threads, 0, s>>>(a, per);
+          return hipGetLastError();
+        }
+      }
       k_cols_pipe<T, N, E, W, PS><<<(unsigned)(groups * SPF), threads, 0, s>>>(a, per);
       return hipGetLastError();
     }
@@ -416,7 +430,7 @@ void add_col(Registry<T>& r) {
                               S::LDS, HOIST ? 1 : 0};
 }
 
-template <typename T, int N, int E, int WT>
+template <typename T, int N, int E, int WT, int HOISTH = HOIST_NONE>
 void add_col_pipe(Registry<T>& r) {
   using S = ColShape<T, N, E, WT>;
   constexpr int lg = ilog2c(N);
@@ -425,8 +439,8 @@ void add_col_pipe(Registry<T>& r) {
   constexpr int lds = S::LDS + TwlLen<N, E>::value * (int)sizeof(cx<T>) +
                       kPipeMaxPer * (int)sizeof(PipeField);
   if constexpr (lds <= kLdsBudget)
-    r.cols[lg][k++] = ColVar<T>{&launch_cols_pipe<T, N, E, S::W, S::PS>, E, S::W, S::THREADS,
-                                lds, 2};
+    r.cols[lg][k++] = ColVar<T>{&launch_cols_pipe<T, N, E, S::W, S::PS, HOISTH>, E, S::W,
+                                S::THREADS, lds, HOISTH != HOIST_NONE ? 3 : 2};
 }
 
 template <typename T, int N>
@@ -456,6 +470,8 @@ void register_n(Registry<T>& r) {
       add_col_pipe<T, N, 32, 8>(r);
       add_col_pipe<T, N, 16, 8>(r);
       add_col_pipe<T, N, 32, 4>(r);
+      add_col_pipe<T, N, 32, 8, HOIST_FACTOR>(r);
+      add_col_pipe<T, N, 16, 8, HOIST_PHASE>(r);
       add_row_pipe<T, N, 32>(r);
     } else {
       add_row<T, N, 16>(r);

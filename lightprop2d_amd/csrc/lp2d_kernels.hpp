@@ -912,6 +912,16 @@ __device__ __forceinline__ cx<float> mid_fast(const ColMid& m, cx<float> v, int 
   return mk(out ? 0.0f : r.re, out ? 0.0f : r.im);
 }
 
+// The FP64 part of mid_fast at signed frequency index sy: the fractional phase of
+// Kz*z and the pupil test (same operations, same order).
+__device__ __forceinline__ float mid_phase(const ColMid& m, int sy, bool& out) {
+#pragma clang fp contract(off)
+  const double fy = (double)sy * m.inv_ndL;
+  const double fy2 = fy * fy;
+  out = m.fx2 + fy2 > m.ap;
+  return kz_frac_fast(m.c0 - fy2, m.z);
+}
+
 // One field of a pipelined column group, staged in LDS at kernel start so that the
 // field loop reads its pointers and optics without global loads (a global load there
 // would wait behind the prefetch and the previous field's stores).
@@ -957,7 +967,19 @@ inline bool pipe_eligible(const PassArgs<T>& a, int mode) {
          !(a.flags & (F_SMASK0 | F_SMASK1));
 }
 
-template <typename T, int N, int E, int W, int PS>
+// HOISTH: the strip's transfer function is computed once, before the field loop, for
+// every field of the group (all fields of the call share one set of optics: a strided
+// batch without per-field tables).
+//   HOIST_FACTOR: exp(-2*pi*i*fr) x pupil held in registers (2E floats); the product
+//                 equals mid_fast's up to the sign of a zero.
+//   HOIST_PHASE:  only the FP64 part, the fractional phase fr and the pupil test, held
+//                 in registers (E floats and one bit mask); each field evaluates sin/cos
+//                 and the product is mid_fast's.
+// Measured per call (pipelined 8-column strips): 1024^2 x32 with 32 elements per thread
+// 118.9 us, factor 113.9, phase 121.8; 256^2 x64 with 16 elements 22.6 us, phase 20.2,
+// factor 22.2; 512^2 x64 with 16: 66.3, phase 62.0, factor 62.7.
+enum : int { HOIST_NONE = 0, HOIST_FACTOR = 1, HOIST_PHASE = 2 };
+template <typename T, int N, int E, int W, int PS, int HOISTH = HOIST_NONE>
 __device__ __forceinline__ void cols_pipe_body(const PassArgs<T>& a, int b0, int b1, int x0,
                                                cx<T>* lds, const cx<T>* twl,
                                                const PipeField* pf, int tid) {
@@ -985,6 +1007,27 @@ __device__ __forceinline__ void cols_pipe_body(const PassArgs<T>& a, int b0, int
   // off the loop so that the loop head is only ever reached in that state (merged
   // with the loop entry the wait would cover the stores as well).
   cx<T> cur[E], nxt[E];
+  constexpr bool HF = HOISTH == HOIST_FACTOR, HP = HOISTH == HOIST_PHASE;
+  static_assert(!HP || E <= 32, "one mask bit per element");
+  cx<T> hm[HF ? E : 1];
+  float hfr[HP ? E : 1];
+  uint32_t hout = 0;
+  if constexpr (HF || HP) {
+    if (mid) {
+      const ColMid cm = col_mid(pf[0].o, flags, sx);
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        bool out;
+        const float fr = mid_phase(cm, t + TL * m - (m >= E / 2 ? N : 0), out);
+        if constexpr (HF) {
+          hm[m] = mk(out ? 0.0f : __builtin_amdgcn_cosf(fr), out ? 0.0f : __builtin_amdgcn_sinf(fr));
+        } else {
+          hfr[m] = fr;
+          hout |= (uint32_t)out << m;
+        }
+      }
+    }
+  }
   auto fetch = [&](int bf) {
     const LP2D_GLB cx<T>* __restrict__ src = glb(uniform_ptr((const cx<T>*)pf[bf - b0].src)) + xoff;
 #pragma unroll
@@ -995,7 +1038,18 @@ __device__ __forceinline__ void cols_pipe_body(const PassArgs<T>& a, int b0, int
     for (int m = 0; m < E; ++m) cur[m] = nxt[m];
     if (b + 1 < b1) fetch(b + 1);  // uniform branch
     line_fft_l<T, N, E, false>(cur, t, acc, twl);
-    if (mid) {
+    if (HF && mid) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) cur[m] = cmulc(cur[m], hm[HF ? m : 0]);
+    } else if (HP && mid) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const float fr = hfr[HP ? m : 0];
+        const cx<float> r = cmulc(cur[m], mk(__builtin_amdgcn_cosf(fr), __builtin_amdgcn_sinf(fr)));
+        const bool out = (hout >> m) & 1u;
+        cur[m] = mk(out ? 0.0f : r.re, out ? 0.0f : r.im);
+      }
+    } else if (mid) {
       ColMid cm = col_mid(pf[b - b0].o, flags, sx);
       // opaque per field: otherwise the per-element frequencies (the same for every
       // field of a strip) are hoisted out of the field loop and held for all E

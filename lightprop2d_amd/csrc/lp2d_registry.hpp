@@ -40,7 +40,7 @@ struct SmallVar {
   int E, threads, lds;
 };
 
-constexpr int kMaxVariants = 12;
+constexpr int kMaxVariants = 16;
 constexpr int kLogMax = 13;  // n <= 8192
 
 template <typename T>
