@@ -38,7 +38,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--autotune", action="store_true", help="pick kernel variants first")
     p.add_argument("--no-extra", action="store_true",
-                   help="skip the uneven-batch and speckle-chain legs")
+                   help="skip the extra workloads (uneven batch, speckle chain, large "
+                        "fields, generators, resampling, mode bases)")
     return p.parse_args()
 
 
@@ -190,10 +191,43 @@ def extra_legs(torch, P, L, lib, dev):
                     "in place",
         "value": round(8 * 10 / el, 1), "unit": "field-propagations/s",
         "ms_per_chain": round(el * 1e3, 4)}
+    del u
+    out["large_fields"] = large_field_leg(torch, P, L, lib, dev)
     out["generators"] = generator_leg(torch, dev)
     out["resample"] = resample_leg(torch, P, dev)
     out["modes"] = modes_leg(torch, P, dev)
     return out
+
+
+def large_field_leg(torch, P, L, lib, dev):
+    """Field-to-field propagation of 8 fields of 4096^2 complex64 (the field size of the
+    multi-plane stack: area 2 mm, 800 nm, 2 mm steps), 1 GiB per sweep: whole-call rate
+    and the per-sweep kernel times from the tracer."""
+    n, B, area, wl, z = 4096, 8, 2000e-4, 800e-7, 2000e-4
+    g = torch.Generator(device="cpu").manual_seed(5)
+    u = torch.randn((B, n, n), dtype=torch.complex64, generator=g).to(dev)
+    out = torch.empty_like(u)
+    for _ in range(2):
+        P.propagate(u, z, area, wl, out=out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 10
+    for _ in range(reps):
+        P.propagate(u, z, area, wl, out=out)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / reps
+    L.check(lib.lp2d_trace_enable(1))
+    P.propagate(u, z, area, wl, out=out)
+    torch.cuda.synchronize()
+    L.check(lib.lp2d_trace_enable(0))
+    sweeps = trace_stats(lib, L)
+    alg = 16.0 * B * n * n
+    return {"workload": "8 fields 4096^2 complex64, area 2 mm, 800 nm, z 2 mm, field to field",
+            "value": round(B / el, 1), "unit": "field-propagations/s",
+            "ms_per_call": round(el * 1e3, 4),
+            "sweeps_us": {k: round(ms * 1e3 / c, 1) for k, (c, ms) in sweeps.items()},
+            "end_to_end_GBs": round(alg / el / 1e9, 1),
+            "end_to_end_frac": round(alg / el / 1e9 / HBM_PEAK_GBS, 4)}
 
 
 MFMA_FP64_PEAK_TFS = 78.6  # MI355X dense FP64 matrix peak

@@ -257,36 +257,48 @@ __device__ __forceinline__ void line_fft(cx<T> (&v)[E], int t, const Acc& acc,
 // the lanes of one wave then start in different banks, and stay 16-byte aligned).  A
 // butterfly reads its R-1 twiddles at one per-lane base plus compile-time offsets, and
 // the inverse transform uses the same table conjugated.
+//
+// CL (compact last pass): when the last pass has radix R <= 4, its table holds only
+// w_N^k (one entry per row k) and the butterfly forms w^{2k} = w^k w^k and
+// w^{3k} = w^{2k} w^k in registers (a rounding or two more than the table).  For
+// n = 4096 with 32 elements per thread this shrinks the tables from 7232 to 2112
+// entries, which lets a 4-column strip and its twiddles share the LDS.
 constexpr int twl_radix(int N, int E, int ns) { return N / ns >= E ? E : N / ns; }
-constexpr int twl_off(int N, int E, int NS) {
+constexpr bool twl_compact(int N, int E, int ns, bool CL) {
+  return CL && ns > 1 && ns * twl_radix(N, E, ns) == N && twl_radix(N, E, ns) <= 4;
+}
+constexpr int twl_off(int N, int E, int NS, bool CL = false) {
   int off = 0;
   for (int ns = 1; ns < NS; ns *= twl_radix(N, E, ns))
-    if (ns > 1) off += ns * (twl_radix(N, E, ns) + 2);
+    if (ns > 1) off += twl_compact(N, E, ns, CL) ? ns : ns * (twl_radix(N, E, ns) + 2);
   return off;
 }
-template <int N, int E>
+template <int N, int E, bool CL = false>
 struct TwlLen {
-  static constexpr int value = twl_off(N, E, N) > 0 ? twl_off(N, E, N) : 1;
+  static constexpr int value = twl_off(N, E, N, CL) > 0 ? twl_off(N, E, N, CL) : 1;
 };
 
 // Fill the per-pass tables from the concatenated global table (size L at offset L-2).
-template <typename T, int N, int E, int THREADS, int NS = 1>
+template <typename T, int N, int E, int THREADS, int NS = 1, bool CL = false>
 __device__ __forceinline__ void twl_fill(cx<T>* twl, const cx<T>* __restrict__ tw, int tid) {
   if constexpr (NS < N) {
     constexpr int R = twl_radix(N, E, NS);
-    if constexpr (NS > 1) {
-      constexpr int L = NS * R, P = R + 2, OFF = twl_off(N, E, NS);
+    if constexpr (twl_compact(N, E, NS, CL)) {
+      constexpr int L = NS * R, OFF = twl_off(N, E, NS, CL);
+      for (int i = tid; i < NS; i += THREADS) twl[OFF + i] = tw[(L - 2) + i];
+    } else if constexpr (NS > 1) {
+      constexpr int L = NS * R, P = R + 2, OFF = twl_off(N, E, NS, CL);
       for (int i = tid; i < L; i += THREADS) {
         const int k = i / R, r = i % R;
         twl[OFF + k * P + r] = tw[(L - 2) + k * r];
       }
     }
-    twl_fill<T, N, E, THREADS, NS * R>(twl, tw, tid);
+    twl_fill<T, N, E, THREADS, NS * R, CL>(twl, tw, tid);
   }
 }
 
 // The Stockham passes above with the twiddles read from the LDS tables.
-template <typename T, int N, int E, bool INV, int NS, typename Acc>
+template <typename T, int N, int E, bool INV, int NS, bool CL = false, typename Acc>
 __device__ __forceinline__ void stockham_l(cx<T> (&v)[E], int t, const Acc& acc,
                                            const cx<T>* twl) {
   constexpr int REM = N / NS;
@@ -301,8 +313,16 @@ __device__ __forceinline__ void stockham_l(cx<T> (&v)[E], int t, const Acc& acc,
     cx<T> a[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) a[r] = v[q + r * B];
-    if constexpr (NS > 1) {
-      const cx<T>* tp = twl + (twl_off(N, E, NS) + k * (R + 2));
+    if constexpr (twl_compact(N, E, NS, CL)) {
+      // w^k from the table; w^{2k}, w^{3k} by products (R <= 4)
+      cx<T> w[R > 1 ? R : 2];
+      w[1] = twl[twl_off(N, E, NS, CL) + k];
+      if constexpr (R > 2) w[2] = cmul(w[1], w[1]);
+      if constexpr (R > 3) w[3] = cmul(w[2], w[1]);
+#pragma unroll
+      for (int r = 1; r < R; ++r) a[r] = INV ? cmulc(a[r], w[r]) : cmul(a[r], w[r]);
+    } else if constexpr (NS > 1) {
+      const cx<T>* tp = twl + (twl_off(N, E, NS, CL) + k * (R + 2));
 #pragma unroll
       for (int r = 1; r < R; ++r) a[r] = INV ? cmulc(a[r], tp[r]) : cmul(a[r], tp[r]);
     }
@@ -321,14 +341,14 @@ __device__ __forceinline__ void stockham_l(cx<T> (&v)[E], int t, const Acc& acc,
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = acc.ld(t + TL * m);
     __syncthreads();
-    stockham_l<T, N, E, INV, NS * R>(v, t, acc, twl);
+    stockham_l<T, N, E, INV, NS * R, CL>(v, t, acc, twl);
   }
 }
 
-template <typename T, int N, int E, bool INV, typename Acc>
+template <typename T, int N, int E, bool INV, bool CL = false, typename Acc>
 __device__ __forceinline__ void line_fft_l(cx<T> (&v)[E], int t, const Acc& acc,
                                            const cx<T>* twl) {
-  stockham_l<T, N, E, INV, 1>(v, t, acc, twl);
+  stockham_l<T, N, E, INV, 1, CL>(v, t, acc, twl);
 }
 
 }  // namespace lp2d

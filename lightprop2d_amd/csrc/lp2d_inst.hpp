@@ -92,10 +92,10 @@ __global__ __launch_bounds__((N / E) * W) LP2D_OCC void k_cols(PassArgs<T> a, in
 // Pipelined column pass: grid (N/W strips) x (field groups of `per` <= kPipeMaxPer
 // fields); the twiddle tables of the strip's passes and the group's field pointers
 // and optics are staged in LDS once per workgroup.
-template <typename T, int N, int E, int W, int PS, int HOISTH = HOIST_NONE>
+template <typename T, int N, int E, int W, int PS, int HOISTH = HOIST_NONE, bool CL = false>
 __global__ __launch_bounds__((N / E) * W) LP2D_OCC void k_cols_pipe(PassArgs<T> a, int per) {
   __shared__ cx<T> lds[W * PaddedLen<N, PS>::value];
-  __shared__ cx<T> twl[TwlLen<N, E>::value];
+  __shared__ cx<T> twl[TwlLen<N, E, CL>::value];
   __shared__ PipeField pf[kPipeMaxPer];
   constexpr int THREADS = (N / E) * W;
   static_assert(THREADS >= kPipeMaxPer, "one thread per staged field");
@@ -106,10 +106,10 @@ __global__ __launch_bounds__((N / E) * W) LP2D_OCC void k_cols_pipe(PassArgs<T> 
   const int x0 = (blk - grp * SPF) * W;
   const int b0 = grp * per;
   const int b1 = cmin(a.batch, b0 + per);
-  twl_fill<T, N, E, THREADS>(twl, a.tw, threadIdx.x);
+  twl_fill<T, N, E, THREADS, 1, CL>(twl, a.tw, threadIdx.x);
   pipe_stage_fields(a, b0, b1, pf, threadIdx.x);
   __syncthreads();
-  cols_pipe_body<T, N, E, W, PS, HOISTH>(a, b0, b1, x0, lds, twl, pf, threadIdx.x);
+  cols_pipe_body<T, N, E, W, PS, HOISTH, CL>(a, b0, b1, x0, lds, twl, pf, threadIdx.x);
 }
 
 // Whole field in LDS, pipelined over groups of `per` <= kPipeMaxPer fields.
@@ -323,7 +323,7 @@ hipError_t launch_cols(const PassArgs<T>& a, int mode, hipStream_t s) {
 // multipliers, other evanescent modes, single-direction passes) runs the plain class
 // of the same shape.  Each workgroup takes `per` fields so that the grid is about one
 // round of resident workgroups.
-template <typename T, int N, int E, int W, int PS, int HOISTH = HOIST_NONE>
+template <typename T, int N, int E, int W, int PS, int HOISTH = HOIST_NONE, bool CL = false>
 hipError_t launch_cols_pipe(const PassArgs<T>& a, int mode, hipStream_t s) {
   const bool ok = pipe_eligible(a, mode);
   // the block-tiled layout is only defined for strips of kTileCols columns, and only
@@ -340,10 +340,10 @@ hipError_t launch_cols_pipe(const PassArgs<T>& a, int mode, hipStream_t s) {
       int slots = 0;
       if constexpr (HOISTH != HOIST_NONE) {
         if (hoist)
-          slots = cached_slots(slot[1], (const void*)&k_cols_pipe<T, N, E, W, PS, HOISTH>, threads);
+          slots = cached_slots(slot[1], (const void*)&k_cols_pipe<T, N, E, W, PS, HOISTH, CL>, threads);
       }
       if (!hoist)
-        slots = cached_slots(slot[0], (const void*)&k_cols_pipe<T, N, E, W, PS>, threads);
+        slots = cached_slots(slot[0], (const void*)&k_cols_pipe<T, N, E, W, PS, HOIST_NONE, CL>, threads);
       int groups = slots / SPF;
       if (groups < 1) groups = 1;
       if (groups > a.batch) groups = a.batch;
@@ -352,11 +352,11 @@ hipError_t launch_cols_pipe(const PassArgs<T>& a, int mode, hipStream_t s) {
       groups = (a.batch + per - 1) / per;
       if constexpr (HOISTH != HOIST_NONE) {
         if (hoist) {
-          k_cols_pipe<T, N, E, W, PS, HOISTH><<<(unsigned)(groups * SPF), threads, 0, s>>>(a, per);
+          k_cols_pipe<T, N, E, W, PS, HOISTH, CL><<<(unsigned)(groups * SPF), threads, 0, s>>>(a, per);
           return hipGetLastError();
         }
       }
-      k_cols_pipe<T, N, E, W, PS><<<(unsigned)(groups * SPF), threads, 0, s>>>(a, per);
+      k_cols_pipe<T, N, E, W, PS, HOIST_NONE, CL><<<(unsigned)(groups * SPF), threads, 0, s>>>(a, per);
       return hipGetLastError();
     }
   }
@@ -430,16 +430,16 @@ void add_col(Registry<T>& r) {
                               S::LDS, HOIST ? 1 : 0};
 }
 
-template <typename T, int N, int E, int WT, int HOISTH = HOIST_NONE>
+template <typename T, int N, int E, int WT, int HOISTH = HOIST_NONE, bool CL = false>
 void add_col_pipe(Registry<T>& r) {
   using S = ColShape<T, N, E, WT>;
   constexpr int lg = ilog2c(N);
   int& k = r.ncols[lg];
   if (k >= kMaxVariants) return;
-  constexpr int lds = S::LDS + TwlLen<N, E>::value * (int)sizeof(cx<T>) +
+  constexpr int lds = S::LDS + TwlLen<N, E, CL>::value * (int)sizeof(cx<T>) +
                       kPipeMaxPer * (int)sizeof(PipeField);
   if constexpr (lds <= kLdsBudget)
-    r.cols[lg][k++] = ColVar<T>{&launch_cols_pipe<T, N, E, S::W, S::PS, HOISTH>, E, S::W,
+    r.cols[lg][k++] = ColVar<T>{&launch_cols_pipe<T, N, E, S::W, S::PS, HOISTH, CL>, E, S::W,
                                 S::THREADS, lds, HOISTH != HOIST_NONE ? 3 : 2};
 }
 
@@ -472,6 +472,9 @@ void register_n(Registry<T>& r) {
       add_col_pipe<T, N, 32, 4>(r);
       add_col_pipe<T, N, 32, 8, HOIST_FACTOR>(r);
       add_col_pipe<T, N, 16, 8, HOIST_PHASE>(r);
+      // the compact twiddle table lets 4-column strips of 4096 points pipeline (the
+      // hoisted factor does not fit in the registers beside them)
+      if constexpr (N >= 4096) add_col_pipe<T, N, 32, 4, HOIST_NONE, true>(r);
       add_row_pipe<T, N, 32>(r);
     } else {
       add_row<T, N, 16>(r);

@@ -979,7 +979,7 @@ inline bool pipe_eligible(const PassArgs<T>& a, int mode) {
 // 118.9 us, factor 113.9, phase 121.8; 256^2 x64 with 16 elements 22.6 us, phase 20.2,
 // factor 22.2; 512^2 x64 with 16: 66.3, phase 62.0, factor 62.7.
 enum : int { HOIST_NONE = 0, HOIST_FACTOR = 1, HOIST_PHASE = 2 };
-template <typename T, int N, int E, int W, int PS, int HOISTH = HOIST_NONE>
+template <typename T, int N, int E, int W, int PS, int HOISTH = HOIST_NONE, bool CL = false>
 __device__ __forceinline__ void cols_pipe_body(const PassArgs<T>& a, int b0, int b1, int x0,
                                                cx<T>* lds, const cx<T>* twl,
                                                const PipeField* pf, int tid) {
@@ -1037,7 +1037,7 @@ __device__ __forceinline__ void cols_pipe_body(const PassArgs<T>& a, int b0, int
 #pragma unroll
     for (int m = 0; m < E; ++m) cur[m] = nxt[m];
     if (b + 1 < b1) fetch(b + 1);  // uniform branch
-    line_fft_l<T, N, E, false>(cur, t, acc, twl);
+    line_fft_l<T, N, E, false, CL>(cur, t, acc, twl);
     if (HF && mid) {
 #pragma unroll
       for (int m = 0; m < E; ++m) cur[m] = cmulc(cur[m], hm[HF ? m : 0]);
@@ -1065,7 +1065,7 @@ __device__ __forceinline__ void cols_pipe_body(const PassArgs<T>& a, int b0, int
 #pragma unroll
       for (int m = 0; m < E; ++m) gst<NT_COL_ST>(so + (lo_n + (unsigned)(TL * N * m)), cur[m]);
     }
-    line_fft_l<T, N, E, true>(cur, t, acc, twl);
+    line_fft_l<T, N, E, true, CL>(cur, t, acc, twl);
     if (scale) {
 #pragma unroll
       for (int m = 0; m < E; ++m) cur[m] = cscale(cur[m], a.scale);
