@@ -5,9 +5,11 @@
 //   cols_body   W columns of one field:  [col FFT] -> [spectral middle] -> [col IFFT]
 //   small_body  a whole n x n field resident in LDS (n <= 128 for c64, <= 64 for c128):
 //               rows -> columns (+middle) -> rows, one HBM read and one HBM write.
+// The *_pipe_body variants run the same work over a sequence of fields or row tiles,
+// loading the next one while the current one is transformed.
 // A field-to-field propagation of a large field is rows(fwd) -> cols(fwd,H,inv) ->
 // rows(inv): three sweeps over the field; the intermediate planes live in the output
-// buffer, so no workspace is needed.
+// buffer, so no workspace is needed (the adjoint can take one for its column pass).
 //
 // The transfer function is generated on chip: fx, fy from the fftfreq index, Kz^2 in
 // FP64, Kz*z in FP64 reduced to its fractional cycle, then sin/cos in the field's
