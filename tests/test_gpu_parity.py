@@ -443,3 +443,28 @@ def test_uneven_batch_per_field_optics(dt):
     for f, d, z, a, w, ap in zip(host_in, dsts, zs, areas, wls, aps):
         exp = oracle.propagate_field(f, a, w, z, aperture_radius=ap if ap else None)
         assert rel_l2(host(d), exp) < tol(dt)
+
+
+@pytest.mark.parametrize("dt,tol", [(np.complex64, 1e-5), (np.complex128, 1e-10)],
+                         ids=["c64", "c128"])
+def test_maximum_size_8192(dt, tol):
+    """The largest supported grid (lp2d_max_n = 8192): one field with a pupil against the
+    complex128 oracle, then the P(z) P(-z) round trip (pupil-free) and Parseval, at the
+    schedules tuned or defaulted for n = 8192."""
+    import oracle
+    from lightprop2d_amd import propagation as P
+    n, area, wl, z = 8192, 4e-1, 800e-7, 0.1
+    ap = 0.05 / wl
+    rs = np.random.RandomState(8)
+    u = cplx(rs, (n, n), dt)
+    print(f"n={n} {dt.__name__} with pupil", flush=True)
+    tu = to_dev(u)
+    got = host(P.propagate(tu, z, area, wl, aperture_radius=ap))
+    exp = oracle.propagate_field(u, area, wl, z, aperture_radius=ap)
+    assert rel_l2(got, exp) < tol
+    del got, exp
+    print(f"n={n} {dt.__name__} round trip", flush=True)
+    v = P.propagate(tu, z, area, wl)
+    assert float((v.abs() ** 2).sum()) == pytest.approx(float((tu.abs() ** 2).sum()), rel=1e-5)
+    w = P.propagate(v, -z, area, wl)
+    assert rel_l2(host(w), u) < (4e-6 if dt == np.complex64 else 1e-12)
