@@ -400,8 +400,10 @@ void add_row(Registry<T>& r) {
   constexpr int lg = ilog2c(N);
   int& k = r.nrows[lg];
   if (k >= kMaxVariants) return;
-  // (taller gradient tiles, for reading a block-tiled plane in place, measured slower:
-  // a 1024-thread workgroup per CU stalls the CU at every exchange barrier)
+  // (taller gradient tiles measured slower: 8 rows, for reading a block-tiled plane in
+  // place, because a 1024-thread workgroup per CU stalls the CU at every exchange
+  // barrier; 4 rows at 1024 (512 threads), reading the adjoint's workspace in 256-byte
+  // pieces, by 5% on the gradient pass)
   r.rows[lg][k++] = RowVar<T>{&launch_rows<T, N, E, S::LINES, S::PS>,
                               &launch_rows_grad<T, N, E, S::LINES, S::PS, (E <= 8)>, E,
                               S::LINES, S::THREADS, S::LDS, S::LINES, 0};
