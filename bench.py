@@ -278,19 +278,27 @@ def modes_leg(torch, P, dev):
 
 
 def resample_leg(torch, P, dev):
-    """Beam2D.expand / coarse kernels on 8 fields of complex64: 1024^2 -> 2048^2 (edge
-    replication) and 2048^2 -> 1024^2 (2x2 block mean).  Algorithmic bytes = one read
-    of the input plus one write of the output; the numpy equivalent of one field is
-    timed beside it."""
+    """Beam2D.expand / coarse / crop / lens_image kernels on 8 fields of complex64:
+    1024^2 -> 2048^2 (edge replication), 2048^2 -> 1024^2 (2x2 block mean), crop to
+    half the area at 1024 points (bilinear 2x zoom of the central 512^2), lens_image at
+    magnification 1/2 (l1 = 1, l2 = 2: the central 512^2 read, chirped).  Algorithmic
+    bytes = one read of the samples used plus one write of the output; the numpy
+    equivalent of one field's expand + coarse is timed beside it."""
     B, n = 8, 1024
+    area, wl = 400e-4, 632e-7
     g = torch.Generator(device="cpu").manual_seed(3)
     u = torch.randn((B, n, n), dtype=torch.complex64, generator=g).to(dev)
     big = P.expand(u, 2 * n)
     res = {}
+    quarter = (n // 2) ** 2
     for name, call, alg in (("expand_1024_to_2048", lambda: P.expand(u, 2 * n),
                              8 * B * (n * n + 4 * n * n)),
                             ("coarse_2048_to_1024", lambda: P.coarse(big, 2),
-                             8 * B * (4 * n * n + n * n))):
+                             8 * B * (4 * n * n + n * n)),
+                            ("crop_half_area_1024", lambda: P.crop(u, area, area / 2),
+                             8 * B * (quarter + n * n)),
+                            ("lens_image_m0.5_1024", lambda: P.lens_image(u, area, wl, 1.0, 2.0),
+                             8 * B * (quarter + n * n))):
         call()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -305,7 +313,8 @@ def resample_leg(torch, P, dev):
     fb = np.pad(f, n // 2, mode="edge")
     fb.reshape((n, 2, n, 2)).mean((1, 3))
     res["numpy_one_field_us"] = round((time.perf_counter() - t0) * 1e6, 1)
-    res["workload"] = "8 fields 1024^2 complex64, expand x2 and coarse(2); device-resident"
+    res["workload"] = ("8 fields 1024^2 complex64: expand x2, coarse(2), crop(area/2), "
+                       "lens_image(l1=1, l2=2); device-resident")
     return res
 
 

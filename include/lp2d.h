@@ -167,6 +167,34 @@ int lp2d_expand(int n, int n_out, int batch, int dtype, const void* in, void* ou
 int lp2d_coarse(int n, int mean_order, int batch, int dtype, const void* in, void* out,
                 void* stream);
 
+/* Bilinear resampling on a regular grid: scipy's interp2d(x, x, field, kind='linear')
+ * (scipy < 1.14) evaluated at ascending queries, the call Beam2D.crop (beam2d.py:406-435)
+ * and Beam2D.lens_image (beam2d.py:513-536) make.  It is FITPACK's degree-1 spline
+ * with a breakpoint at every sample: a query outside [t_0, t_{len-1}] is clamped to
+ * the edge, the interval is the last l <= len-2 with t_l <= q, and the weights are
+ * (t_{l+1} - q)/(t_{l+1} - t_l) and (q - t_l)/(t_{l+1} - t_l), all in FP64.
+ * Both axes use the same coordinates.  Source: the win_len x win_len window at
+ * (win_off, win_off) of each n x n field; window sample k sits at
+ * t_k = (k + src_i0) * src_dx.  Output sample j is taken at
+ * q_j = ((dst_sign * j + dst_i0) * dst_dx) * dst_scale.  The caller orders the queries
+ * as interp2d sorts them; dst_sign = -1 covers a negative dst_scale.
+ * out (batch, n_out, n_out) = scale * interp * exp(-i * phase_coef * (x^2 + y^2)),
+ * with x_j = (j + grid_i0) * grid_dl (phase_coef = 0: no chirp).  in and out must not
+ * overlap. */
+typedef struct lp2d_interp_desc {
+  int32_t win_off, win_len; /* window of the source field, both axes (win_len >= 2) */
+  int32_t src_i0;           /* t_k = (k + src_i0) * src_dx */
+  int32_t dst_i0, dst_sign; /* q_j = ((dst_sign * j + dst_i0) * dst_dx) * dst_scale */
+  int32_t grid_i0;          /* chirp coordinate x_j = (j + grid_i0) * grid_dl */
+  double src_dx;            /* > 0 */
+  double dst_dx, dst_scale;
+  double scale;             /* output factor */
+  double phase_coef;        /* chirp coefficient [rad / cm^2] */
+  double grid_dl;
+} lp2d_interp_desc;
+int lp2d_interp_linear(int n, int n_out, int batch, int dtype, const void* in, void* out,
+                       const lp2d_interp_desc* desc, void* stream);
+
 /* ---------------------------------------------------------------- mode bases */
 /* Mode-basis projection (Beam2D.deconstruct_by_modes 'scalar', beam2d.py:563-603):
  * coeffs[b, k] = sum_p modes[k, p] * field[b, window(p)] -- the reference's

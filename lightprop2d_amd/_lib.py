@@ -41,7 +41,7 @@ EXPORTED = [
     "lp2d_batch_run", "lp2d_batch_destroy", "lp2d_variant_count",
     "lp2d_set_variant", "lp2d_get_variant", "lp2d_variant_name", "lp2d_trace_enable",
     "lp2d_trace_collect", "lp2d_moments", "lp2d_set_chunk", "lp2d_get_chunk",
-    "lp2d_profile", "lp2d_expand", "lp2d_coarse", "lp2d_modes_project",
+    "lp2d_profile", "lp2d_expand", "lp2d_coarse", "lp2d_interp_linear", "lp2d_modes_project",
     "lp2d_modes_project_ws_size", "lp2d_modes_project_ws",
     "lp2d_modes_construct",
 ]
@@ -82,6 +82,13 @@ class lp2d_profile_params(C.Structure):
                 ("thresh", C.c_double), ("x0", C.c_double), ("y0", C.c_double),
                 ("dx", C.c_double), ("dy", C.c_double), ("slits", C.c_double),
                 ("binning", C.c_int32), ("reserved", C.c_int32)]
+
+
+class lp2d_interp_desc(C.Structure):
+    _fields_ = [("win_off", C.c_int32), ("win_len", C.c_int32), ("src_i0", C.c_int32),
+                ("dst_i0", C.c_int32), ("dst_sign", C.c_int32), ("grid_i0", C.c_int32),
+                ("src_dx", C.c_double), ("dst_dx", C.c_double), ("dst_scale", C.c_double),
+                ("scale", C.c_double), ("phase_coef", C.c_double), ("grid_dl", C.c_double)]
 
 
 class lp2d_trace_stat(C.Structure):
@@ -129,6 +136,7 @@ _SIGS = {
     "lp2d_profile": (_I, [_I, _I, _I, _P, _P, C.POINTER(lp2d_profile_params), _P, _P, _I, _P]),
     "lp2d_expand": (_I, [_I, _I, _I, _I, _P, _P, _P]),
     "lp2d_coarse": (_I, [_I, _I, _I, _I, _P, _P, _P]),
+    "lp2d_interp_linear": (_I, [_I, _I, _I, _I, _P, _P, C.POINTER(lp2d_interp_desc), _P]),
     "lp2d_modes_project": (_I, [_I, _I, _I, _I, _I, _P, _P, _P, _P]),
     "lp2d_modes_project_ws_size": (C.c_longlong, [_I, _I, _I, _I]),
     "lp2d_modes_project_ws": (_I, [_I, _I, _I, _I, _I, _P, _P, _P, _P, C.c_longlong, _P]),

@@ -9,8 +9,11 @@ Differences, all deliberate:
   ``use_gpu`` flag is accepted and ignored (the HIP path always runs);
 * ``complex_bits=64`` computes in complex64 kernels, 128 in complex128 kernels;
 * ``init_field`` is copied to the device rather than aliased;
-* ``crop`` / ``lens_image`` resample without scipy's removed ``interp2d``
-  (see ``lightprop2d_amd.resample``).
+* ``crop`` / ``lens_image`` resample with a HIP kernel that computes what scipy's
+  removed ``interp2d(kind='linear')`` computed (``propagation.interp_linear``), and the
+  field keeps the beam's precision (the reference's becomes complex128 there, since
+  interp2d returns float64); ``lens_image`` leaves ``X`` / ``Y`` alone, as the
+  reference's cupy path does (its numpy path scales them in place through an alias).
 """
 from __future__ import annotations
 
@@ -189,6 +192,26 @@ class Beam2D:
         self.area_size = area_size
         self._construct_grids()
         self.field = field
+        self.spectrum = self._fft2(self.field)
+
+    def crop(self, area_size: float, npoints: int = 0):
+        """beam2d.py:406-435: keep the centred samples with |X| < area_size/2 and
+        interpolate them bilinearly onto npoints (kept when 0) samples of the new area;
+        spectrum = fft2(field)."""
+        field = P.crop(self.field, self.area_size, area_size, npoints)
+        if npoints:
+            self.npoints = npoints
+        self.area_size = area_size
+        self._construct_grids()
+        self.field = field
+        self.spectrum = self._fft2(self.field)
+
+    def lens_image(self, f: float, l1: float, l2: float):
+        """beam2d.py:513-536: the image between conjugate planes l1 before and l2 after
+        the lens -- the field resampled at X * (-l1/l2) (sorted, so never flipped),
+        times -l1/l2 and the chirp exp(-i k0 (l1 + l2) r^2 / (2 l2^2)); f is unused,
+        as in the reference.  spectrum = fft2(field)."""
+        self.field = P.lens_image(self.field, self.area_size, self.wl, l1, l2)
         self.spectrum = self._fft2(self.field)
 
     def coarse(self, mean_order: int = 1):

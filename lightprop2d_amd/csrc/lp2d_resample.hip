@@ -1,9 +1,12 @@
 // Grid resampling of Beam2D.expand / Beam2D.coarse (lightprop2d/beam2d.py:375-404,
-// 437-460) behind lp2d_expand / lp2d_coarse.  Both are one pass over the fields,
-// bound by HBM: expand reads n^2 and writes n_out^2 elements, coarse reads n^2 and
-// writes (n/m)^2.
+// 437-460) behind lp2d_expand / lp2d_coarse, and the bilinear interpolation of
+// Beam2D.crop / Beam2D.lens_image (beam2d.py:406-435, 513-536) behind
+// lp2d_interp_linear.  Each is one pass over the fields, bound by HBM: expand reads n^2
+// and writes n_out^2 elements, coarse reads n^2 and writes (n/m)^2, the interpolation
+// gathers at most the window and writes n_out^2.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdlib.h>
 
 #include "../../include/lp2d.h"
 
@@ -108,6 +111,106 @@ __global__ void k_coarse2(const float2* __restrict__ in, float2* __restrict__ ou
   __builtin_nontemporal_store(o, (v4f*)(out + (long long)row * no + I0));
 }
 
+// ---------------------------------------------------------------- interpolation
+// interp2d(kind='linear') on a rectangular grid is FITPACK's degree-1 spline with a
+// knot at every sample, evaluated by fpbisp: the argument is clamped to
+// [t_0, t_{len-1}], the interval is the last l <= len-2 with t_l <= q, and fpbspl's
+// basis values are f*(t_{l+1} - q) and f*(q - t_l) with f = 1/(t_{l+1} - t_l).  The
+// knots t_k and queries q_j are formed as integer index times spacing, the way the
+// reference builds X (beam2d.py:302-304), so they carry the reference's rounding.
+struct AxisTap {
+  int l;          // window sample of the left tap
+  double h0, h1;  // weights of samples l and l + 1
+};
+
+__device__ inline double knot(const lp2d_interp_desc& d, int k) {
+  return (double)(k + d.src_i0) * d.src_dx;
+}
+
+__device__ inline AxisTap axis_tap(const lp2d_interp_desc& d, int j) {
+  double q = ((double)(d.dst_sign * j + d.dst_i0) * d.dst_dx) * d.dst_scale;
+  const double tb = knot(d, 0), te = knot(d, d.win_len - 1);
+  if (q < tb) q = tb;
+  if (q > te) q = te;
+  const int last = d.win_len - 2;
+  int l = (int)floor((q - tb) / d.src_dx);
+  l = l < 0 ? 0 : (l > last ? last : l);
+  // the estimate is off by at most one sample where q sits on a knot; settle it on the
+  // knots themselves (both loops are bounded by the window)
+  while (l < last && knot(d, l + 1) <= q) ++l;
+  while (l > 0 && knot(d, l) > q) --l;
+  const double tl = knot(d, l), tr = knot(d, l + 1);
+  const double f = 1.0 / (tr - tl);
+  return AxisTap{l, f * (tr - q), f * (q - tl)};
+}
+
+// Chirp factor exp(-i * c * x^2) of one grid coordinate; the chirp of a pixel is the
+// product of its column's and its row's factors.
+__device__ inline double2 chirp_factor(const lp2d_interp_desc& d, int j) {
+  const double x = (double)(j + d.grid_i0) * d.grid_dl;
+  double sn, cs;
+  sincos(-(x * x) * d.phase_coef, &sn, &cs);
+  return make_double2(cs, sn);
+}
+
+// One thread per output column, kInterpRows output rows each; grid (column chunks of
+// 256, row chunks, fields).  The row taps and row chirp factors are computed once per
+// workgroup into LDS.  The value is accumulated in FP64 in fpbisp's order (x tap
+// outer, y tap inner), then scaled and chirped in FP64 and rounded once to the field's
+// precision.
+constexpr int kInterpRows = 16;
+constexpr int kInterpThreads = 256;
+
+template <typename C>
+__global__ __launch_bounds__(kInterpThreads) void k_interp_linear(
+    const C* __restrict__ in, C* __restrict__ out, int n, int n_out, lp2d_interp_desc d) {
+  __shared__ AxisTap ty_s[kInterpRows];
+  __shared__ double2 cy_s[kInterpRows];
+  const bool chirp = d.phase_coef != 0.0;
+  const int i0 = blockIdx.y * kInterpRows;
+  const int rows = n_out - i0 < kInterpRows ? n_out - i0 : kInterpRows;
+  if ((int)threadIdx.x < rows) {
+    ty_s[threadIdx.x] = axis_tap(d, i0 + (int)threadIdx.x);
+    if (chirp) cy_s[threadIdx.x] = chirp_factor(d, i0 + (int)threadIdx.x);
+  }
+  __syncthreads();
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_out) return;
+  const int b = blockIdx.z;
+  const AxisTap tx = axis_tap(d, j);
+  const double2 cx = chirp ? chirp_factor(d, j) : make_double2(1.0, 0.0);
+  const C* win = in + ((long long)b * n + d.win_off) * n + d.win_off + tx.l;
+  C* dst = out + ((long long)b * n_out + i0) * n_out + j;
+#pragma unroll 4
+  for (int r = 0; r < rows; ++r) {
+    const AxisTap ty = ty_s[r];
+    const C* r0 = win + (long long)ty.l * n;
+    const C* r1 = r0 + n;
+    const C c00 = r0[0], c01 = r1[0], c10 = r0[1], c11 = r1[1];  // c<x tap><y tap>
+    double vr = ((double)c00.x * tx.h0) * ty.h0;
+    double vi = ((double)c00.y * tx.h0) * ty.h0;
+    vr += ((double)c01.x * tx.h0) * ty.h1;
+    vi += ((double)c01.y * tx.h0) * ty.h1;
+    vr += ((double)c10.x * tx.h1) * ty.h0;
+    vi += ((double)c10.y * tx.h1) * ty.h0;
+    vr += ((double)c11.x * tx.h1) * ty.h1;
+    vi += ((double)c11.y * tx.h1) * ty.h1;
+    vr *= d.scale;
+    vi *= d.scale;
+    if (chirp) {
+      const double2 cy = cy_s[r];
+      const double er = cx.x * cy.x - cx.y * cy.y, ei = cx.x * cy.y + cx.y * cy.x;
+      const double t = vr * er - vi * ei;
+      vi = vr * ei + vi * er;
+      vr = t;
+    }
+    C o;
+    o.x = (decltype(o.x))vr;
+    o.y = (decltype(o.y))vi;
+    dst[(long long)r * n_out] = o;
+  }
+}
+
 int prep() {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return lp2d::set_error(LP2D_E_HIP, "hipGetDevice failed");
@@ -161,6 +264,47 @@ extern "C" int lp2d_coarse(int n, int mean_order, int batch, int dtype, const vo
   else
     k_coarse<double, double2><<<grid, 256, 0, s>>>((const double2*)in, (double2*)out, n,
                                                    mean_order);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? LP2D_OK : set_error(LP2D_E_HIP, hipGetErrorString(e));
+}
+
+extern "C" int lp2d_interp_linear(int n, int n_out, int batch, int dtype, const void* in,
+                                  void* out, const lp2d_interp_desc* desc, void* stream) {
+  using lp2d::set_error;
+  if (!desc) return set_error(LP2D_E_ARG, "null interpolation descriptor");
+  const lp2d_interp_desc d = *desc;
+  if (n < 2 || n_out < 1 || n_out > 65536 || batch < 1 || batch > 65535)
+    return set_error(LP2D_E_SIZE, "interp_linear needs n >= 2, 1 <= n_out <= 65536, "
+                                  "1 <= batch <= 65535");
+  if (d.win_len < 2 || d.win_off < 0 || d.win_off > n - d.win_len)
+    return set_error(LP2D_E_SIZE, "interpolation window must hold >= 2 samples inside the field");
+  if (dtype != LP2D_C64 && dtype != LP2D_C128) return set_error(LP2D_E_DTYPE, "unknown dtype");
+  if (!in || !out) return set_error(LP2D_E_ARG, "null pointer");
+  if (d.dst_sign != 1 && d.dst_sign != -1) return set_error(LP2D_E_ARG, "dst_sign must be +1 or -1");
+  const double params[] = {d.src_dx, d.dst_dx, d.dst_scale, d.scale, d.phase_coef, d.grid_dl};
+  for (double v : params)
+    if (!(v - v == 0.0)) return set_error(LP2D_E_ARG, "non-finite interpolation parameter");
+  if (!(d.src_dx > 0.0)) return set_error(LP2D_E_ARG, "src_dx must be positive");
+  // index products stay inside int32: |dst_sign * j + dst_i0|, |k + src_i0|, |j + grid_i0|
+  const long long lim = 0x3fffffffLL;
+  if (llabs((long long)d.dst_i0) + n_out > lim || llabs((long long)d.src_i0) + d.win_len > lim ||
+      llabs((long long)d.grid_i0) + n_out > lim)
+    return set_error(LP2D_E_ARG, "index offset out of range");
+  const size_t es = dtype == LP2D_C64 ? 8 : 16;
+  const char* ib = (const char*)in;
+  const char* ob = (const char*)out;
+  if (ib < ob + es * (size_t)batch * n_out * n_out && ob < ib + es * (size_t)batch * n * n)
+    return set_error(LP2D_E_ARG, "in and out must not overlap");
+  if (int rc = prep()) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 grid((unsigned)((n_out + kInterpThreads - 1) / kInterpThreads),
+                  (unsigned)((n_out + kInterpRows - 1) / kInterpRows), (unsigned)batch);
+  if (dtype == LP2D_C64)
+    k_interp_linear<float2><<<grid, kInterpThreads, 0, s>>>((const float2*)in, (float2*)out, n,
+                                                            n_out, d);
+  else
+    k_interp_linear<double2><<<grid, kInterpThreads, 0, s>>>((const double2*)in, (double2*)out,
+                                                             n, n_out, d);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? LP2D_OK : set_error(LP2D_E_HIP, hipGetErrorString(e));
 }

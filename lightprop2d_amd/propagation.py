@@ -322,6 +322,74 @@ def coarse(field: torch.Tensor, mean_order: int) -> torch.Tensor:
     return out
 
 
+def interp_linear(field: torch.Tensor, n_out: int, desc: "L.lp2d_interp_desc") -> torch.Tensor:
+    """Bilinear resampling (..., n, n) -> (..., n_out, n_out): scipy's
+    interp2d(kind='linear') on the regular grid ``desc`` describes, with its output
+    scale and chirp (the interp2d calls of beam2d.py:426-434, 525-535).  Computed in
+    FP64 and rounded once to the field's dtype."""
+    lib = _lib()
+    n, batch, dt = _field_geom(field)
+    n_out = int(n_out)
+    out = torch.empty(field.shape[:-2] + (n_out, n_out), dtype=field.dtype, device=field.device)
+    with torch.cuda.device(field.device):
+        L.check(lib.lp2d_interp_linear(n, n_out, batch, dt, _ptr(field), _ptr(out),
+                                       C.byref(desc), _stream(field)), "interp_linear")
+    return out
+
+
+def crop_desc(n: int, area_size: float, area_new: float, npoints: int = 0):
+    """Window and grids of Beam2D.crop (beam2d.py:419-432): the centred samples with
+    |X| < area_new / 2 of the n-point grid of side area_size (an even count 2h around
+    n // 2), taken at the new grid X' = arange(-m // 2, m // 2) * area_new / m
+    (m = npoints, or n when 0).  Returns (desc, m)."""
+    n = int(n)
+    m = int(npoints) if npoints else n
+    dL = area_size / n
+    X = np.arange(-n // 2, n // 2, 1) * dL            # beam2d.py:302-303
+    h = len(X[np.abs(X) < area_new / 2]) // 2
+    if h < 1:
+        raise ValueError(f"crop to {area_new} keeps {2 * h} samples per axis; interp2d "
+                         "needs at least 2")
+    if m < 1:
+        raise ValueError("npoints must be positive")
+    Nc = n // 2
+    d = L.lp2d_interp_desc(win_off=Nc - h, win_len=2 * h, src_i0=Nc - h + (-n // 2),
+                           dst_i0=-m // 2, dst_sign=1, grid_i0=0, src_dx=dL,
+                           dst_dx=area_new / m, dst_scale=1.0, scale=1.0, phase_coef=0.0,
+                           grid_dl=0.0)
+    return d, m
+
+
+def crop(field: torch.Tensor, area_size: float, area_new: float, npoints: int = 0):
+    """Beam2D.crop's resampling (beam2d.py:406-435) of (..., n, n) fields of side
+    area_size onto npoints (n when 0) samples of side area_new."""
+    d, m = crop_desc(field.shape[-1], area_size, area_new, npoints)
+    return interp_linear(field, m, d)
+
+
+def lens_image_desc(n: int, area_size: float, wl: float, l1: float, l2: float):
+    """Grids of Beam2D.lens_image (beam2d.py:524-535): the field on X = arange(-n // 2,
+    n // 2) * dL taken at sorted(X * s), s = -l1 / l2, times s and the chirp
+    exp(-i k0 (l1 + l2) / (2 l2^2) (X^2 + Y^2)), k0 = 1 / wl."""
+    n = int(n)
+    dL = area_size / n
+    k0 = 1 / wl
+    s = - l1 / l2
+    i0 = -n // 2
+    # interp2d sorts the queries X * s: for s < 0 that is X read backwards
+    sign, q0 = (1, i0) if s >= 0 else (-1, n - 1 + i0)
+    return L.lp2d_interp_desc(win_off=0, win_len=n, src_i0=i0, dst_i0=q0, dst_sign=sign,
+                              grid_i0=i0, src_dx=dL, dst_dx=dL, dst_scale=s, scale=s,
+                              phase_coef=k0 * (l1 + l2) / 2 / l2 ** 2, grid_dl=dL)
+
+
+def lens_image(field: torch.Tensor, area_size: float, wl: float, l1: float, l2: float):
+    """Beam2D.lens_image's imaging between conjugate planes (beam2d.py:513-536) of
+    (..., n, n) fields of side area_size."""
+    n = field.shape[-1]
+    return interp_linear(field, n, lens_image_desc(n, area_size, wl, l1, l2))
+
+
 def mode_matrix(modes, dtype, device) -> torch.Tensor:
     """(nmodes, nb*nb) contiguous device tensor of `dtype` from a list of flattened
     modes (pyMMF layout, beam2d.py:538-561), a 2-D array or a tensor."""

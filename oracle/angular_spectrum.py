@@ -30,6 +30,14 @@ How each function is pinned (see tests/test_oracle.py):
 * ``iprofile``              |u|^2.
 * ``propagate_field_adjoint``  <A u, g> = <u, A^H g> dot-product test, central
                              finite differences on the phase gradient.
+* ``interp2d_linear``       scipy's RectBivariateSpline(kx=1, ky=1, s=0), which runs the
+                             same FITPACK fit (regrid_smth) and evaluation (bispev) that
+                             interp2d(kind='linear') ran; exact on bilinear functions;
+                             samples reproduced at the nodes; edge clamping.
+* ``OracleBeam2D.crop``     the window copied unchanged when the new grid lands on the
+                             old samples; ``lens_image``: l1 = l2 gives minus the field
+                             shifted by one sample (interp2d's sorted queries) times the
+                             closed-form chirp, l1 = -l2 the field unchanged.
 
 No function is "parity unpinned".
 """
@@ -45,7 +53,7 @@ __all__ = [
     "Grids", "construct_grids", "fft2", "ifft2", "transfer_function",
     "lens_phase", "circular_aperture", "OracleBeam2D",
     "propagate_field", "propagate_field_adjoint", "propagate_stack",
-    "propagate_stack_grad",
+    "propagate_stack_grad", "interp2d_linear",
 ]
 
 # Units, centimetre base (beam2d.py:37-41)
@@ -119,6 +127,53 @@ def circular_aperture(Kx: np.ndarray, Ky: np.ndarray, radius: float) -> np.ndarr
     oracle.profiles.round_hole.)
     """
     return (Kx ** 2 + Ky ** 2) <= radius ** 2
+
+
+def _fitpack_linear_taps(t, q):
+    """FITPACK fpbisp / fpbspl for degree 1 on knots at the samples t (ascending): the
+    argument clamped to [t_0, t_-1], the interval the last l <= len-2 with t_l <= q,
+    basis values f*(t_{l+1} - q) and f*(q - t_l), f = 1/(t_{l+1} - t_l)."""
+    q = np.clip(q, t[0], t[-1])
+    l = np.minimum(np.searchsorted(t, q, side="right") - 1, len(t) - 2)
+    f = 1.0 / (t[l + 1] - t[l])
+    return l, f * (t[l + 1] - q), f * (q - t[l])
+
+
+def interp2d_linear(x, y, z, xq, yq):
+    """``scipy.interpolate.interp2d(x, y, z, kind='linear')(xq, yq)`` for data on a
+    rectangular grid, as it ran before scipy 1.14 removed it (the calls of Beam2D.crop
+    and Beam2D.lens_image, beam2d.py:426-427, 431, 525-526, 532).
+
+    interp2d fitted FITPACK's regrid_smth with kx = ky = 1 and s = 0 -- the piecewise
+    bilinear interpolant with a breakpoint at every sample -- and evaluated it with
+    bisplev at the SORTED queries (mergesort; an image is never flipped), where a query
+    outside [x_0, x_-1] takes the edge value.  z[i, j] is the value at (x[j], y[i]);
+    the result is (len(yq), len(xq)).  Real and imaginary parts are interpolated
+    separately by the reference (with real weights, the same as complex arithmetic),
+    in FP64, summed in fpbisp's order (x tap outer, y tap inner).
+    """
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    z = np.asarray(z)
+    if len(x) < 2 or len(y) < 2:
+        raise ValueError("a linear spline needs at least 2 samples per axis")
+    jx = np.argsort(x, kind="mergesort")
+    jy = np.argsort(y, kind="mergesort")
+    x, y, z = x[jx], y[jy], z[np.ix_(jy, jx)]
+    xq = np.sort(np.atleast_1d(np.asarray(xq, dtype=np.float64)), kind="mergesort")
+    yq = np.sort(np.atleast_1d(np.asarray(yq, dtype=np.float64)), kind="mergesort")
+    lx, hx0, hx1 = _fitpack_linear_taps(x, xq)
+    ly, hy0, hy1 = _fitpack_linear_taps(y, yq)
+    zc = z.astype(np.result_type(z.dtype, np.float64))
+
+    def tap(a, b):
+        return zc[np.ix_(ly + b, lx + a)]
+
+    out = (tap(0, 0) * hx0[None, :]) * hy0[:, None]
+    out = out + (tap(0, 1) * hx0[None, :]) * hy1[:, None]
+    out = out + (tap(1, 0) * hx1[None, :]) * hy0[:, None]
+    out = out + (tap(1, 1) * hx1[None, :]) * hy1[:, None]
+    return out
 
 
 class OracleBeam2D:
@@ -222,6 +277,34 @@ class OracleBeam2D:
         f = np.vstack((np.repeat(f[0].reshape((1, -1)), d_npoints, axis=0), f,
                        np.repeat(f[-1].reshape((1, -1)), d_npoints, axis=0)))
         self.field = f
+        self.spectrum = fft2(self.field)
+
+    def crop(self, area_size, npoints=0):
+        # beam2d.py:419-435: the centred samples with |X| < area_size/2 (an even count
+        # 2n), interpolated onto the new grid (npoints kept when 0).  interp2d returns
+        # float64, so the field becomes complex128 here, as in the reference.
+        old_X = self.X[np.abs(self.X) < area_size / 2]
+        Nc = self.npoints // 2
+        n = len(old_X) // 2
+        old_X = self.X[Nc - n:Nc + n]
+        if npoints:
+            self.npoints = npoints
+        self.area_size = area_size
+        cropped_profile = self.field[Nc - n:Nc + n, Nc - n:Nc + n]
+        self._construct_grids()
+        self.field = interp2d_linear(old_X, old_X, cropped_profile, self.X, self.X)
+        self.spectrum = fft2(self.field)
+
+    def lens_image(self, f, l1, l2):
+        # beam2d.py:524-536 (f is unused there too).  The reference's numpy path scales
+        # self.X in place through the alias X = self._np(self.X); its cupy path (.get())
+        # copies.  This follows the copy, leaving the grids alone.
+        scale_factor = - l1 / l2
+        X = self.X.copy()
+        phase = - (self.X ** 2 + self.Y ** 2) * self.k0 * (l1 + l2) / 2 / l2 ** 2
+        Xq = X * scale_factor
+        self.field = scale_factor * interp2d_linear(X, X, self.field, Xq, Xq) * \
+            (np.cos(phase) + 1j * np.sin(phase))
         self.spectrum = fft2(self.field)
 
     def coarse(self, mean_order=1):

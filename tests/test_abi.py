@@ -130,3 +130,64 @@ def test_argument_errors_of_the_generator_resample_and_mode_entry_points(lib):
     assert lib.lp2d_modes_project_ws(64, 63, 3, 1, 0, p, p, p, w, 1 << 20, None) == _lib.LP2D_E_SIZE
     assert lib.lp2d_modes_construct(64, 32, 0, 1, 0, p, p, p, None) == _lib.LP2D_E_SIZE
     assert lib.lp2d_modes_construct(64, 32, 3, 1, 2, p, p, p, None) == _lib.LP2D_E_DTYPE
+
+
+def test_argument_errors_of_the_interpolation_entry_point(lib):
+    from lightprop2d_amd import _lib
+    p, q = C.c_void_p(16), C.c_void_p(1 << 40)
+
+    def desc(**kw):
+        d = dict(win_off=0, win_len=64, src_i0=-32, dst_i0=-32, dst_sign=1, grid_i0=-32,
+                 src_dx=1e-4, dst_dx=1e-4, dst_scale=1.0, scale=1.0, phase_coef=0.0,
+                 grid_dl=1e-4)
+        d.update(kw)
+        return C.byref(_lib.lp2d_interp_desc(**d))
+
+    f = lib.lp2d_interp_linear
+    assert f(64, 64, 1, 0, p, q, None, None) == _lib.LP2D_E_ARG
+    assert f(64, 0, 1, 0, p, q, desc(), None) == _lib.LP2D_E_SIZE
+    assert f(64, 64, 1, 0, p, q, desc(win_len=1), None) == _lib.LP2D_E_SIZE
+    assert f(64, 64, 1, 0, p, q, desc(win_off=1), None) == _lib.LP2D_E_SIZE
+    assert b"window" in lib.lp2d_last_error()
+    assert f(64, 64, 1, 5, p, q, desc(), None) == _lib.LP2D_E_DTYPE
+    assert f(64, 64, 1, 0, None, q, desc(), None) == _lib.LP2D_E_ARG
+    assert f(64, 64, 1, 0, p, q, desc(dst_sign=0), None) == _lib.LP2D_E_ARG
+    assert f(64, 64, 1, 0, p, q, desc(src_dx=0.0), None) == _lib.LP2D_E_ARG
+    assert f(64, 64, 1, 0, p, q, desc(scale=float("nan")), None) == _lib.LP2D_E_ARG
+    assert f(64, 64, 1, 0, p, q, desc(phase_coef=float("inf")), None) == _lib.LP2D_E_ARG
+    assert f(64, 64, 1, 0, p, C.c_void_p(16 + 8 * 64), desc(), None) == _lib.LP2D_E_ARG
+    assert b"overlap" in lib.lp2d_last_error()
+
+
+def test_interpolation_grids_are_the_reference_coordinates():
+    """The descriptors of crop / lens_image reproduce the reference's coordinate arrays
+    bit for bit: the window's knots old_X and the new grid X (beam2d.py:419-432), and
+    the sorted queries X * (-l1/l2) (beam2d.py:524-532)."""
+    import numpy as np
+    from lightprop2d_amd import propagation as P
+
+    def knots(d):
+        return (np.arange(d.win_len) + d.src_i0) * d.src_dx
+
+    def queries(d, m):
+        return ((d.dst_sign * np.arange(m) + d.dst_i0) * d.dst_dx) * d.dst_scale
+
+    for n, area, frac, npts in ((256, 200e-4, 0.6, 0), (512, 400e-4, 0.5, 128), (64, 1.0, 0.3, 50)):
+        d, m = P.crop_desc(n, area, frac * area, npts)
+        X = np.arange(-n // 2, n // 2, 1) * (area / n)
+        old = X[np.abs(X) < frac * area / 2]
+        h, Nc = len(old) // 2, n // 2
+        assert np.array_equal(knots(d), X[Nc - h:Nc + h])
+        assert d.win_off == Nc - h and d.win_len == 2 * h
+        mm = npts or n
+        Xn = np.arange(-mm // 2, mm // 2, 1) * (frac * area / mm)
+        assert m == mm and np.array_equal(queries(d, m), Xn)
+    with pytest.raises(ValueError):
+        P.crop_desc(64, 1.0, 1.0 / 64)
+    for n, l1, l2 in ((256, 1.0, 2.0), (128, -1.5, 1.0), (64, 0.0, 1.0), (64, 3.0, 0.7)):
+        area, wl = 200e-4, 632e-7
+        d = P.lens_image_desc(n, area, wl, l1, l2)
+        X = np.arange(-n // 2, n // 2, 1) * (area / n)
+        assert np.array_equal(knots(d), X)
+        assert np.array_equal(queries(d, n), np.sort(X * (-l1 / l2), kind="mergesort"))
+        assert d.scale == -l1 / l2 and d.grid_i0 == -n // 2 and d.grid_dl == area / n

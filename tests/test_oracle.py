@@ -328,6 +328,103 @@ def test_coarse_is_block_mean():
     np.testing.assert_allclose(c.field, np.full((32, 32), 2 - 1j), rtol=0, atol=1e-15)
 
 
+def test_interp2d_linear_matches_fitpack():
+    # interp2d(kind='linear') on a rectangular grid ran FITPACK's regrid_smth (kx = ky =
+    # 1, s = 0) and bispev; RectBivariateSpline runs the same two routines (its axes are
+    # (y, x) for z[y, x]).  Uneven knots, queries outside, on and between the knots,
+    # given unsorted (interp2d sorts them).
+    from scipy.interpolate import RectBivariateSpline
+    from oracle.angular_spectrum import interp2d_linear
+    rs = np.random.RandomState(21)
+    x = np.cumsum(rs.uniform(0.2, 1.5, 17)) - 6.0
+    y = np.cumsum(rs.uniform(0.1, 2.0, 11)) - 4.0
+    z = rs.standard_normal((11, 17)) + 1j * rs.standard_normal((11, 17))
+    xq = np.concatenate([rs.uniform(x[0] - 3, x[-1] + 3, 40), x[[0, 5, 16]]])
+    yq = np.concatenate([rs.uniform(y[0] - 3, y[-1] + 3, 30), y[[0, 4, 10]]])
+    rs.shuffle(xq)
+    rs.shuffle(yq)
+    got = interp2d_linear(x, y, z, xq, yq)
+    xs, ys = np.sort(xq), np.sort(yq)
+    exp = (RectBivariateSpline(y, x, z.real, kx=1, ky=1, s=0)(ys, xs)
+           + 1j * RectBivariateSpline(y, x, z.imag, kx=1, ky=1, s=0)(ys, xs))
+    assert got.shape == (len(yq), len(xq))
+    np.testing.assert_allclose(got, exp, rtol=1e-13, atol=1e-13)
+    # a transposed or unsorted evaluation would not agree
+    assert np.abs(interp2d_linear(x, y, z, xq, yq)[:, ::-1] - exp).max() > 0.1
+
+
+def test_interp2d_linear_closed_forms():
+    # exact on a + b x + c y + d x y, the samples themselves at the knots, the edge
+    # samples outside the grid
+    from oracle.angular_spectrum import interp2d_linear
+    x = np.arange(-8, 8) * 0.25
+    y = np.arange(-5, 7) * 0.5
+
+    def fxy(X, Y):
+        return (1.5 - 2j) + 0.75 * X + (0.25 + 1j) * Y - 0.5j * X * Y
+
+    z = fxy(x[None, :], y[:, None])
+    xq = np.linspace(x[0], x[-1], 53)
+    yq = np.linspace(y[0], y[-1], 29)
+    np.testing.assert_allclose(interp2d_linear(x, y, z, xq, yq), fxy(xq[None, :], yq[:, None]),
+                               rtol=0, atol=1e-13)
+    np.testing.assert_allclose(interp2d_linear(x, y, z, x, y), z, rtol=0, atol=1e-15)
+    out = interp2d_linear(x, y, z, [-100.0, 100.0], [-100.0, 100.0])
+    np.testing.assert_allclose(out, z[np.ix_([0, -1], [0, -1])], rtol=0, atol=1e-15)
+
+
+def test_crop_keeps_the_window_on_the_old_samples():
+    # crop to 2h samples of the same spacing: the new grid is the old window, so the
+    # field is that window (interp2d at its own knots)
+    rs = np.random.RandomState(22)
+    n, area = 64, 128e-4
+    f0 = rs.standard_normal((n, n)) + 1j * rs.standard_normal((n, n))
+    b = OracleBeam2D(area, n, 632e-7, init_field=f0)
+    dL = area / n
+    b.crop(33 * dL, 32)        # |X| < 16.5 dL keeps X = -16 .. 16 dL: 33 samples, n = 16
+    assert b.npoints == 32 and abs(b.dL - 33 * dL / 32) < 1e-18
+    c = OracleBeam2D(area, n, 632e-7, init_field=f0)
+    c.crop(32 * dL, 32)        # |X| < 16 dL: 31 samples -> the 30 around the centre
+    assert c.field.dtype == np.complex128 and c.field.shape == (32, 32)
+    # new X = -16 .. 15 dL on the old spacing: X[1:31] are the window's knots
+    # (f0[17:47]); X = -16 and 15 dL clamp to the window's edges
+    np.testing.assert_allclose(c.field[1:31, 1:31], f0[17:47, 17:47], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(c.field[0, 1:31], f0[17, 17:47], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(c.field[31, 31], f0[46, 46], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(c.spectrum, np.fft.fft2(c.field), rtol=0, atol=1e-12)
+    # npoints = 0 keeps the number of points (a 2x zoom here): |X| < 16 dL keeps the 30
+    # samples X = -15 .. 14 dL (f0[17:47]); the new even samples sit at X = -16 .. 15 dL,
+    # the two ends clamped to the window's edge
+    d = OracleBeam2D(area, n, 632e-7, init_field=f0)
+    d.crop(area / 2)
+    assert d.npoints == n and d.field.shape == (n, n)
+    np.testing.assert_allclose(d.field[2:62:2, 2:62:2], f0[17:47, 17:47], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(d.field[0, 2:62:2], f0[17, 17:47], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(d.field[62, 62], f0[46, 46], rtol=0, atol=1e-14)
+    # the odd samples are the means of their neighbours
+    np.testing.assert_allclose(d.field[3, 2:62:2], 0.5 * (f0[17, 17:47] + f0[18, 17:47]),
+                               rtol=0, atol=1e-14)
+
+
+def test_lens_image_closed_forms():
+    # l1 = l2: scale -1, interp2d's sorted queries are X shifted by one sample (the last
+    # clamped), so the image is not flipped; chirp exp(-i k0 r^2 / l).  l1 = -l2: scale
+    # +1, no chirp, the field unchanged.
+    rs = np.random.RandomState(23)
+    n, area, wl, l = 32, 64e-4, 632e-7, 0.5
+    f0 = rs.standard_normal((n, n)) + 1j * rs.standard_normal((n, n))
+    b = OracleBeam2D(area, n, wl, init_field=f0)
+    b.lens_image(1.0, l, l)
+    k = np.minimum(np.arange(n) + 1, n - 1)
+    r2 = b.X[None, :] ** 2 + b.X[:, None] ** 2
+    exp = -f0[np.ix_(k, k)] * np.exp(-1j * r2 / wl / l)
+    np.testing.assert_allclose(b.field, exp, rtol=0, atol=1e-12)
+    assert np.abs(b.X - (np.arange(-n // 2, n // 2) * area / n)).max() == 0
+    c = OracleBeam2D(area, n, wl, init_field=f0)
+    c.lens_image(1.0, -l, l)
+    np.testing.assert_allclose(c.field, f0, rtol=0, atol=1e-14)
+
+
 def test_threaded_baseline_matches_the_oracle():
     # the multi-threaded CPU baseline computes the same propagation
     from oracle.angular_spectrum import propagate_field_threaded
