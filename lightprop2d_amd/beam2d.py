@@ -9,6 +9,9 @@ Differences, all deliberate:
   ``use_gpu`` flag is accepted and ignored (the HIP path always runs);
 * ``complex_bits=64`` computes in complex64 kernels, 128 in complex128 kernels;
 * ``init_field`` is copied to the device rather than aliased;
+* ``npoints`` is a power of two from 2 to 8192 (the device FFT); ``expand``,
+  ``coarse`` and ``crop`` check the new grid and raise ``Lp2dError`` before they
+  change any state;
 * ``crop`` / ``lens_image`` resample with a HIP kernel that computes what scipy's
   removed ``interp2d(kind='linear')`` computed (``propagation.interp_linear``), and the
   field keeps the beam's precision (the reference's becomes complex128 there, since
@@ -35,6 +38,13 @@ cm = 1
 m = 100
 
 ArrayLike = Union[np.ndarray, torch.Tensor]
+
+
+def _check_grid(npoints: int) -> None:
+    """The device FFT runs powers of two from 2 to 8192 points."""
+    if npoints < 2 or npoints > 8192 or npoints & (npoints - 1):
+        raise Lp2dError(f"npoints = {npoints}: the device FFT needs a power of two "
+                        "from 2 to 8192")
 
 
 @dataclass
@@ -184,9 +194,9 @@ class Beam2D:
     def expand(self, area_size: float):
         """beam2d.py:375-404: enlarge the area at constant dL, replicating the edge rows
         and columns of the field; spectrum = fft2(field)."""
-        old_npoints = self.npoints
         npoints = int(self.npoints * area_size / self.area_size)
         npoints -= npoints % 2
+        _check_grid(npoints)
         field = P.expand(self.field, npoints)
         self.npoints = npoints
         self.area_size = area_size
@@ -198,6 +208,7 @@ class Beam2D:
         """beam2d.py:406-435: keep the centred samples with |X| < area_size/2 and
         interpolate them bilinearly onto npoints (kept when 0) samples of the new area;
         spectrum = fft2(field)."""
+        _check_grid(npoints or self.npoints)
         field = P.crop(self.field, self.area_size, area_size, npoints)
         if npoints:
             self.npoints = npoints
@@ -217,7 +228,9 @@ class Beam2D:
     def coarse(self, mean_order: int = 1):
         """beam2d.py:437-460: block-average the field over mean_order x mean_order
         (npoints //= mean_order); spectrum = fft2(field)."""
-        field = P.coarse(self.field, mean_order)
+        if mean_order >= 1 and self.npoints % mean_order == 0:
+            _check_grid(self.npoints // mean_order)
+        field = P.coarse(self.field, mean_order)     # ValueError for a non-divisor
         self.npoints //= mean_order
         self._construct_grids()
         self.field = field

@@ -315,7 +315,7 @@ const RowVar<T>& grad_variant(int lg) {
 
 // F_TILED when the selected schedules can hand the intermediate plane between the
 // sweeps over in the block-tiled layout: the pipelined complex64 column class with
-// strips of kTileCols, row tiles of whole kTileRows-row blocks, and (grad) the
+// strips of tile_cols(n), row tiles of whole tile_rows(n)-row blocks, and (grad) the
 // adjoint's gradient pass reading it (whole blocks again, unless it stores nothing).
 template <typename T>
 int tiled_flag(const PassArgs<T>& cols_a, int lg, bool grad = false, bool grad_store = true) {
@@ -323,12 +323,13 @@ int tiled_flag(const PassArgs<T>& cols_a, int lg, bool grad = false, bool grad_s
   const int n = 1 << lg;
   const ColVar<T>& cv = reg<T>()->cols[lg][g_sel[dt<T>()][1][lg]];
   const RowVar<T>& rv = reg<T>()->rows[lg][g_sel[dt<T>()][0][lg]];
-  if (!(cv.kind & 2) || cv.W != kTileCols || (n / cv.E) % kTileRows != 0) return 0;
+  const int tr = tile_rows(n), tc = tile_cols(n);
+  if (!(cv.kind & 2) || cv.W != tc || (n / cv.E) % tr != 0) return 0;
   if (!pipe_eligible(cols_a, MODE_FWD | MODE_INV)) return 0;
-  if (rv.lines % kTileRows != 0 || (n / rv.E) % kTileCols != 0) return 0;
+  if (rv.lines % tr != 0 || (n / rv.E) % tc != 0) return 0;
   if (grad) {
     const RowVar<T>& gv = grad_variant<T>(lg);
-    if ((gv.grad_lines % kTileRows != 0 && grad_store) || (n / gv.E) % kTileCols != 0) return 0;
+    if ((gv.grad_lines % tr != 0 && grad_store) || (n / gv.E) % tc != 0) return 0;
   }
   return F_TILED;
 }

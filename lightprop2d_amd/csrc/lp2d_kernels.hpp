@@ -38,14 +38,45 @@ enum : int {
   F_TILED = 1 << 14,     // the intermediate plane between the sweeps is block-tiled
 };
 
-// Block-tiled intermediate plane (F_TILED): rows in blocks of 8, each block stored as
-// 8-column strips of 8 x 8 elements, i.e. element (y, x) of an n x n plane at
-//   (y/8)*8n + (x/8)*64 + (y%8)*8 + x%8.
-// A block of 8 rows occupies the same addresses as in row-major order, so a row pass
-// over whole blocks converts in place; a column strip of width 8 is then 512-byte
-// chunks instead of 64-byte pieces of 1024 different rows, and every 128-byte line is
-// written whole by one store instruction.
-constexpr int kTileRows = 8, kTileCols = 8;
+// Block-tiled intermediate plane (F_TILED): rows in blocks of R = tile_rows(n), each
+// block stored as C = tile_cols(n)-column strips of R x C elements, i.e. element (y, x)
+// of an n x n plane at
+//   (y/R)*R*n + (x/C)*R*C + (y%R)*C + x%C.
+// A block of R rows occupies the same addresses as in row-major order, so a row pass
+// over whole blocks converts in place; a column strip of width C then reads R*C
+// contiguous elements per block instead of C-element pieces of R different rows.  The
+// geometry follows each length's schedules (row tiles of R rows, strips of C columns):
+//   8 x 8 up to 2048 points: 512-byte chunks, every 128-byte line written whole by one
+//         store instruction (at 2048 the 4-column strips leave the plane row-major);
+//   2 x 4 at 4096: each 2-row block stays contiguous, so the 2-row passes keep their
+//         access pattern, and the 4-column strips read 64-byte chunks instead of 32.
+// (4 x 4 tiles at 2048 sped the column pass up by 25 us of 240, slowed each row pass
+// by about 10 us and doubled the adjoint's one-row gradient pass: not used.)
+constexpr int tile_rows(int n) { return n >= 4096 ? 2 : 8; }
+constexpr int tile_cols(int n) { return n >= 4096 ? 4 : 8; }
+
+// Offset of (y, x) in the block-tiled layout of an N-point plane.
+template <int N>
+__device__ __forceinline__ unsigned tiled_off(int y, int x) {
+  constexpr unsigned R = tile_rows(N), C = tile_cols(N);
+  const unsigned uy = (unsigned)y, ux = (unsigned)x;
+  return (uy / R) * R * N + (ux / C) * (R * C) + (uy % R) * C + ux % C;
+}
+
+// Thread -> (row l, lane t) for reading a block-tiled plane with row tiles of TR-row
+// blocks: each wave takes TR rows x 64/TR lanes, ordered so that the wave's 64 loads
+// of one element index cover 64 contiguous elements (whole tiles).  Needs TR*TL a
+// multiple of 64 and 64/TR of TC.
+template <int TL, int TR, int TC>
+struct TiledWaveMap {
+  static constexpr bool ok = (TR * TL) % 64 == 0 && (64 / TR) % TC == 0;
+  static __device__ __forceinline__ void map(int tid, int& l, int& t) {
+    const int g = tid / (TR * TL), r = tid - g * (TR * TL);
+    const int w = r >> 6, lam = r & 63;
+    l = g * TR + (lam / TC) % TR;
+    t = w * (64 / TR) + (lam / (TR * TC)) * TC + lam % TC;
+  }
+};
 constexpr int F_SPATIAL = F_MASK0 | F_MASK1 | F_LENS | F_PHASE | F_FIELD_OUT;
 constexpr int F_MID = F_H | F_APERTURE | F_SMASK0 | F_SMASK1 | F_SPEC_OUT;
 
@@ -491,16 +522,15 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
                                           cx<T>* lds, int tid) {
   constexpr int TL = N / E;
   constexpr int L = RowPitch<N, PS, LINES>::value;
+  constexpr int TR = tile_rows(N), TC = tile_cols(N);
+  using WM = TiledWaveMap<TL, TR, TC>;
   // Thread -> (row l, lane t).  Row-major planes: each row's TL threads consecutive.
-  // An inverse pass reading a block-tiled plane: each wave takes 8 rows x 8 lanes, so
-  // every load instruction of the wave reads one whole 8x8 tile (512 contiguous bytes).
+  // An inverse pass reading a block-tiled plane: each wave takes TR rows x 64/TR lanes,
+  // so every load instruction of the wave reads whole tiles (8 x 8: one 512-byte tile).
+  // (The same map for the forward pass of 2 x 4 tiles measured no different.)
   int l = tid / TL, t = tid - (tid / TL) * TL;
-  if constexpr (INV && LINES % kTileRows == 0 && TL % kTileCols == 0) {
-    if (a.flags & F_TILED) {
-      const int g = tid / (kTileRows * TL), r = tid - g * (kTileRows * TL);
-      l = g * kTileRows + ((r & 63) >> 3);
-      t = (r >> 6) * kTileCols + (r & 7);
-    }
+  if constexpr (INV && LINES % TR == 0 && TL % TC == 0 && WM::ok) {
+    if (a.flags & F_TILED) WM::map(tid, l, t);
   }
   const int y = valid_row(y0 + l, y0 + nrows - 1);
   const bool valid = l < nrows;
@@ -508,15 +538,15 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
   // uniform base (SGPRs) + one 32-bit per-lane offset
   const long long ubase = (long long)y0 * N;
   const unsigned lo = (unsigned)((y - y0) * N + t);
-  // block-tiled offset of (y, x = t + TL*m): lo_t + 8*TL*m (y0 a multiple of 8, TL of 8)
+  // block-tiled offset of (y, x = t + TL*m): lo_t + TR*TL*m (y0 a multiple of TR, TL of
+  // TC)
   const bool tiled = (a.flags & F_TILED) != 0;
-  const unsigned lo_t = (unsigned)((l >> 3) * kTileRows * N + (t >> 3) * (kTileRows * kTileCols) +
-                                   (l & 7) * kTileCols + (t & 7));
+  const unsigned lo_t = tiled_off<N>(l, t);
   cx<T> v[E];
   const LP2D_GLB cx<T>* __restrict__ src = field_src(a, b) + ubase;
   if (INV && tiled) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = gld<NT_ROW_LD>(src + (lo_t + (unsigned)(kTileRows * TL * m)));
+    for (int m = 0; m < E; ++m) v[m] = gld<NT_ROW_LD>(src + (lo_t + (unsigned)(tile_rows(N) * TL * m)));
   } else {
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = gld<NT_ROW_LD>(src + (lo + (unsigned)(TL * m)));
@@ -539,7 +569,7 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
     if ((a.flags & F_NO_STORE) == 0) {
       if (!INV && tiled) {
 #pragma unroll
-        for (int m = 0; m < E; ++m) gst<INV ? NT_ROWI_ST : NT_ROW_ST>(dst + (lo_t + (unsigned)(kTileRows * TL * m)), v[m]);
+        for (int m = 0; m < E; ++m) gst<INV ? NT_ROWI_ST : NT_ROW_ST>(dst + (lo_t + (unsigned)(tile_rows(N) * TL * m)), v[m]);
       } else {
 #pragma unroll
         for (int m = 0; m < E; ++m)
@@ -551,7 +581,7 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
     if (!INV && tiled) {
 #pragma unroll
       for (int m = 0; m < E; ++m)
-        gst<INV ? NT_ROWI_ST : NT_ROW_ST>(dst + (lo_t + (unsigned)(kTileRows * TL * m)), mk(v[m].re * sc, v[m].im * sc));
+        gst<INV ? NT_ROWI_ST : NT_ROW_ST>(dst + (lo_t + (unsigned)(tile_rows(N) * TL * m)), mk(v[m].re * sc, v[m].im * sc));
     } else {
 #pragma unroll
       for (int m = 0; m < E; ++m)
@@ -576,32 +606,32 @@ __device__ __forceinline__ void rows_pipe_body(const PassArgs<T>& a, int u0, int
   constexpr int L = RowPitch<N, PS, LINES>::value;
   constexpr int TPF = N / LINES;  // row tiles per field
   static_assert(N % LINES == 0, "whole tiles");
-  static_assert(!TILED || (LINES % kTileRows == 0 && TL % kTileCols == 0), "tiled shape");
+  constexpr int TR = tile_rows(N), TC = tile_cols(N);
+  using WM = TiledWaveMap<TL, TR, TC>;
+  static_assert(!TILED || (LINES % TR == 0 && TL % TC == 0), "tiled shape");
   int l = tid / TL, t = tid - (tid / TL) * TL;
   if constexpr (INV && TILED) {
-    const int g = tid / (kTileRows * TL), r = tid - g * (kTileRows * TL);
-    if constexpr (TL % 16 == 0) {
+    if constexpr (TR == 8 && TC == 8 && TL % 16 == 0) {
       // each wave takes 4 rows x 16 lanes of an 8-row block: every load instruction
       // reads two 256-byte runs of the tiled plane, every store writes whole 128-byte
       // lines of the natural one (with 8 x 8 lanes per wave the two halves of a line
       // come from two waves, and the line is written back twice)
+      const int g = tid / (TR * TL), r = tid - g * (TR * TL);
       const int wv = r >> 6, ln = r & 63;
-      l = g * kTileRows + (ln >> 4) + 4 * (wv & 1);
+      l = g * TR + (ln >> 4) + 4 * (wv & 1);
       t = (ln & 15) + 16 * (wv >> 1);
-    } else {
-      // 8 rows x 8 lanes: every load instruction reads one 8x8 tile
-      l = g * kTileRows + ((r & 63) >> 3);
-      t = (r >> 6) * kTileCols + (r & 7);
+    } else if constexpr (WM::ok) {
+      // TR rows x 64/TR lanes: every load instruction reads whole tiles
+      WM::map(tid, l, t);
     }
   }
   // lane offsets inside a tile of LINES rows: natural (y, x) at y*N + x, tiled see
   // F_TILED; element m of the thread is x = t + TL*m
   const unsigned lo_n = (unsigned)(l * N + t);
-  const unsigned lo_t = (unsigned)((l >> 3) * kTileRows * N + (t >> 3) * (kTileRows * kTileCols) +
-                                   (l & 7) * kTileCols + (t & 7));
+  const unsigned lo_t = tiled_off<N>(l, t);
   constexpr bool LD_T = INV && TILED, ST_T = !INV && TILED;
   const unsigned ld_o = LD_T ? lo_t : lo_n, st_o = ST_T ? lo_t : lo_n;
-  constexpr unsigned LD_S = LD_T ? kTileRows * TL : TL, ST_S = ST_T ? kTileRows * TL : TL;
+  constexpr unsigned LD_S = LD_T ? TR * TL : TL, ST_S = ST_T ? TR * TL : TL;
   LineAcc<T, PS> acc{lds + l * L};
   const bool scale = (a.flags & F_SCALE) != 0;
   auto tile_off = [&](int u) {
@@ -639,8 +669,8 @@ __device__ __forceinline__ void rows_pipe_body(const PassArgs<T>& a, int u0, int
 
 // Thread -> (row l, lane t) of a gradient row tile of LINES rows.  Reading a
 // block-tiled plane, each wave takes all LINES rows x 64/LINES lanes, so every load
-// instruction reads whole 128-byte lines (rows y, y+1 of an 8x8 tile are adjacent);
-// with one row per wave each line would be read in two halves by two waves.
+// instruction reads whole 128-byte lines at 8 x 8 tiles (rows y, y+1 of a tile are
+// adjacent); with one row per wave each line would be read in two halves by two waves.
 template <int TL, int LINES>
 __device__ __forceinline__ void grad_thread_map(int tid, bool tiled, int& l, int& t) {
   l = tid / TL;
@@ -683,17 +713,16 @@ __device__ __forceinline__ void rows_grad_body(const PassArgs<T>& a, int b0, int
     if constexpr (HOIST) ec[m] = grad_fold(a, mk(a.scale, (T)0), 0, y, t + TL * m, N);
     gsum[m] = 0.f;
   }
-  // block-tiled offset of (y, x = t + TL*m) in the field: lo_t + 8*TL*m (TL a multiple
-  // of 8); any row tile height can read it, storing in place needs whole blocks
-  const unsigned lo_t = (unsigned)((y >> 3) * kTileRows * N + (t >> 3) * (kTileRows * kTileCols) +
-                                   (y & 7) * kTileCols + (t & 7));
+  // block-tiled offset of (y, x = t + TL*m) in the field: lo_t + R*TL*m (TL a multiple
+  // of C); any row tile height can read it, storing in place needs whole blocks
+  const unsigned lo_t = tiled_off<N>(y, t);
   for (int b = b0; b < b1; ++b) {
     const long long frow = (long long)b * a.fstride + (long long)y * N;
     cx<T> v[E];
     if (tiled) {
       const LP2D_GLB cx<T>* __restrict__ src = glb(a.src + (long long)b * a.fstride);
 #pragma unroll
-      for (int m = 0; m < E; ++m) v[m] = gld<NT_GRAD>(src + (lo_t + (unsigned)(kTileRows * TL * m)));
+      for (int m = 0; m < E; ++m) v[m] = gld<NT_GRAD>(src + (lo_t + (unsigned)(tile_rows(N) * TL * m)));
     } else {
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = a.src[frow + t + TL * m];
@@ -823,15 +852,14 @@ __device__ __forceinline__ void rows_grad_pipe_body(const PassArgs<T>& a, int b0
     gsum[m] = 0.f;
   }
   const unsigned lo_n = (unsigned)(y * N + t);
-  const unsigned lo_t = (unsigned)((y >> 3) * kTileRows * N + (t >> 3) * (kTileRows * kTileCols) +
-                                   (y & 7) * kTileCols + (t & 7));
+  const unsigned lo_t = tiled_off<N>(y, t);
   cx<T> cur[E], nxt[E], ucur[E], unxt[E];
   auto fetch = [&](int bf) {
     const LP2D_GLB cx<T>* __restrict__ src = glb(a.src + (long long)bf * a.fstride);
     const LP2D_GLB cx<T>* __restrict__ us = glb(a.u_saved + (long long)bf * a.u_sb);
     if (tiled) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) nxt[m] = gld<NT_GRAD>(src + (lo_t + (unsigned)(kTileRows * TL * m)));
+      for (int m = 0; m < E; ++m) nxt[m] = gld<NT_GRAD>(src + (lo_t + (unsigned)(tile_rows(N) * TL * m)));
     } else {
 #pragma unroll
       for (int m = 0; m < E; ++m) nxt[m] = gld<NT_GRAD>(src + (lo_n + (unsigned)(TL * m)));
@@ -990,13 +1018,12 @@ __device__ __forceinline__ void cols_pipe_body(const PassArgs<T>& a, int b0, int
   const int c = tid % W, t = tid / W;
   const int x = x0 + c;
   const int sx = x < N / 2 ? x : x - N;
-  // natural: (y, x) at y*N + x; block-tiled (W == kTileCols): see F_TILED.  Row
-  // y = t + TL*m is at lo + TL*N*m in both (TL a multiple of 8 when tiled).
+  // natural: (y, x) at y*N + x; block-tiled (W == tile_cols(N)): see F_TILED.  Row
+  // y = t + TL*m is at lo + TL*N*m in both (TL a multiple of tile_rows(N) when tiled).
   const bool tiled = (a.flags & F_TILED) != 0;
   const unsigned lo_n = (unsigned)(t * N + c);
-  const unsigned lo =
-      tiled ? (unsigned)((t >> 3) * kTileRows * N + (t & 7) * kTileCols + c) : lo_n;
-  const int xoff = tiled ? (x0 / kTileCols) * (kTileRows * kTileCols) : x0;
+  const unsigned lo = tiled ? tiled_off<N>(t, c) : lo_n;
+  const int xoff = tiled ? (x0 / tile_cols(N)) * (tile_rows(N) * tile_cols(N)) : x0;
   InterAcc<T, PS, W> acc{lds + c};
   const int flags = a.flags;
   const bool mid = (flags & (F_H | F_APERTURE)) != 0;

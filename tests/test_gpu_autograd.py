@@ -246,18 +246,19 @@ def test_config5_full_size_checkpointed():
     assert float((g2.double() * d.double()).sum()) == pytest.approx(fd, rel=1e-3)
 
 
-@pytest.mark.parametrize("n", [256, 512, 1024])
+@pytest.mark.parametrize("n", [256, 512, 1024, 2048, 4096])
 @pytest.mark.parametrize("screen", ["shared", "per_field", "none"])
 def test_adjoint_workspace_matches_oracle(n, screen):
     """The adjoint with a workspace (column pass out of place, block-tiled intermediate
-    read by the gradient schedule) and without one (every pass in grad_in) against the
-    complex128 oracle adjoint, for a shared screen (batch-summed gradient), per-field
-    screens and no screen; batch 5 leaves partial groups in the pipelined passes."""
+    read by the gradient schedule: 8 x 8 tiles up to 1024, row-major at 2048, 2 x 4
+    tiles at 4096) and without one (every pass in grad_in) against the complex128 oracle adjoint,
+    for a shared screen (batch-summed gradient), per-field screens and no screen; batches
+    of 5 (3 from 2048 on) leave partial groups in the pipelined passes."""
     from lightprop2d_amd import propagation as P
     rs = np.random.RandomState(n + len(screen))
     area, wl, z = 300e-4, 632e-7, 0.03
     ap = 0.3 / wl
-    B = 5
+    B = 5 if n <= 1024 else 3
     u = (rs.standard_normal((B, n, n)) + 1j * rs.standard_normal((B, n, n))).astype(np.complex64)
     go = (rs.standard_normal((B, n, n)) + 1j * rs.standard_normal((B, n, n))).astype(np.complex64)
     if screen == "shared":

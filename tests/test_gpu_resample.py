@@ -154,3 +154,20 @@ def test_beam2d_crop_and_lens_image_chain_vs_oracle(bits):
         assert rel_l2(b.field.cpu().numpy(), o.field) < tol, op
         assert rel_l2(b.spectrum.cpu().numpy(), o.spectrum) < tol, op
     assert torch.equal(b.X.cpu(), torch.from_numpy(o.X))
+
+
+def test_resampling_to_an_unsupported_grid_leaves_the_beam_unchanged():
+    """expand to 1.5x the area (64 -> 96 points) and crop to 100 points have no
+    power-of-two FFT on the device, nor does coarse(64) (one point): Lp2dError before
+    any state changes; coarse by a non-divisor raises the reference's ValueError, also
+    before."""
+    from lightprop2d_amd import Beam2D, um, nm
+    from lightprop2d_amd._lib import Lp2dError
+    b = Beam2D(100 * um, 64, 632 * nm, init_field=np.ones((64, 64)), complex_bits=64)
+    f0, s0, x0 = b.field.clone(), b.spectrum.clone(), b.X.clone()
+    for op, args, err in (("expand", (150 * um,), Lp2dError), ("coarse", (3,), ValueError),
+                          ("coarse", (64,), Lp2dError), ("crop", (50 * um, 100), Lp2dError)):
+        with pytest.raises(err):
+            getattr(b, op)(*args)
+        assert b.npoints == 64 and b.area_size == 100 * um
+        assert torch.equal(b.field, f0) and torch.equal(b.spectrum, s0) and torch.equal(b.X, x0)
