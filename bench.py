@@ -40,6 +40,8 @@ def parse():
     p.add_argument("--no-extra", action="store_true",
                    help="skip the extra workloads (uneven batch, speckle chain, large "
                         "fields, generators, resampling, mode bases)")
+    p.add_argument("--leg", choices=sorted(LEGS),
+                   help="run only this extra workload and print its JSON (for profiling)")
     return p.parse_args()
 
 
@@ -355,6 +357,15 @@ def generator_leg(torch, dev):
     return res
 
 
+# extra workloads that run alone with --leg (uniform signature)
+LEGS = {
+    "large_fields": lambda torch, P, L, lib, dev: large_field_leg(torch, P, L, lib, dev),
+    "resample": lambda torch, P, L, lib, dev: resample_leg(torch, P, dev),
+    "modes": lambda torch, P, L, lib, dev: modes_leg(torch, P, dev),
+    "generators": lambda torch, P, L, lib, dev: generator_leg(torch, dev),
+}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -369,6 +380,9 @@ def main():
     lib = L.load()
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.leg:
+        print(json.dumps({args.leg: LEGS[args.leg](torch, P, L, lib, dev)}), flush=True)
+        return
     n, B = args.n, args.batch
     area, wl, z = 400e-4, 632e-7, 50e-4
     if args.autotune:
