@@ -351,7 +351,9 @@ hipError_t launch_cols_pipe(const PassArgs<T>& a, int mode, hipStream_t s) {
       if (per > kPipeMaxPer) per = kPipeMaxPer;
       // at 4096 a field's 1024 strips already take four rounds of the resident
       // workgroups: groups of at most 4 fields measured 800-811 us per 8 fields against
-      // 833-841 us for one group of 8 (2 fields: 797-813 us, 1 field: 830-841 us)
+      // 833-841 us for one group of 8 (2 fields: 797-813 us, 1 field: 830-841 us).  At
+      // 1024, where the groups of 8 fill one round, caps of 4 and 2 took 125 and 131 us
+      // against 115 us.
       if constexpr (N >= 4096) {
         if (per > 4) per = 4;
       }
