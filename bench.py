@@ -130,12 +130,24 @@ def trace_stats(lib, L):
 
 
 def extra_legs(torch, P, L, lib, dev):
-    """Two more workloads, timed the same way (inputs resident, HIP-event kernel times):
-    the uneven batch of 4096 fields (one launch per size bucket and sweep over
-    per-field descriptor tables, captured in a HIP graph), and the 10-step speckle
-    chain at 1024^2.  alg_bytes counts one read and one write of every field (16 B
-    per complex64 element), the least any propagation can move."""
-    out = {}
+    """The other workloads, timed the same way (inputs resident, HIP-event kernel
+    times): the uneven batch of 4096 fields (one launch per size bucket and sweep over
+    per-field descriptor tables, captured in a HIP graph), the 10-step speckle chain at
+    1024^2, large fields, the profile generators, resampling and mode bases.  alg_bytes
+    counts one read and one write of every field (16 B per complex64 element), the
+    least any propagation can move."""
+    out = {"uneven_batch": uneven_leg(torch, P, L, lib, dev),
+           "speckle_chain": speckle_leg(torch, P, dev)}
+    out["large_fields"] = large_field_leg(torch, P, L, lib, dev)
+    out["generators"] = generator_leg(torch, dev)
+    out["resample"] = resample_leg(torch, P, dev)
+    out["modes"] = modes_leg(torch, P, dev)
+    return out
+
+
+def uneven_leg(torch, P, L, lib, dev):
+    """Configuration 3's uneven batch: 4096 fields of three sizes, size-bucketed launches
+    over per-field descriptor tables, replayed from a HIP graph."""
     # uneven batch: 2048 x 128^2 + 1536 x 256^2 + 512 x 512^2, complex64, z ~ U[10 um, 1 mm]
     g = torch.Generator(device="cpu").manual_seed(1)
     sizes = [128] * 2048 + [256] * 1536 + [512] * 512
@@ -166,7 +178,7 @@ def extra_legs(torch, P, L, lib, dev):
     torch.cuda.synchronize()
     el = (time.perf_counter() - t0) / K
     alg = 16.0 * sum(nn * nn for nn in sizes)
-    out["uneven_batch"] = {
+    res = {
         "workload": "4096 fields (2048x128^2 + 1536x256^2 + 512x512^2) complex64, per-field z, "
                     "area 200 um, 532 nm; size-bucketed launches over descriptor tables, "
                     "HIP graph replay",
@@ -176,8 +188,14 @@ def extra_legs(torch, P, L, lib, dev):
         "end_to_end_frac": round(alg / el / 1e9 / HBM_PEAK_GBS, 4),
         "sweeps_us": {k: round(ms * 1e3 / c, 1) for k, (c, ms) in sweeps.items()}}
     del ub, graph, srcs, pools
+    return res
+
+
+def speckle_leg(torch, P, dev):
+    """The 10-step speckle chain at 1024^2 (8 fields, NA 0.2 aperture, in place)."""
     # speckle chain: 1024^2, 10 x [propagate 50 um, NA 0.2 aperture], field to field
     n, area, wl = 1024, 400e-4, 632e-7
+    g = torch.Generator(device="cpu").manual_seed(2)
     u = torch.randn((8, n, n), dtype=torch.complex64, generator=g).to(dev)
     for _ in range(2):
         P.propagate(u, 50e-4, area, wl, aperture_radius=0.2 / wl, out=u)
@@ -188,17 +206,13 @@ def extra_legs(torch, P, L, lib, dev):
             P.propagate(u, 50e-4, area, wl, aperture_radius=0.2 / wl, out=u)
     torch.cuda.synchronize()
     el = (time.perf_counter() - t0) / 5
-    out["speckle_chain"] = {
+    res = {
         "workload": "8 speckle fields 1024^2 complex64, 10 x [propagate 50 um, NA 0.2 aperture], "
                     "in place",
         "value": round(8 * 10 / el, 1), "unit": "field-propagations/s",
         "ms_per_chain": round(el * 1e3, 4)}
     del u
-    out["large_fields"] = large_field_leg(torch, P, L, lib, dev)
-    out["generators"] = generator_leg(torch, dev)
-    out["resample"] = resample_leg(torch, P, dev)
-    out["modes"] = modes_leg(torch, P, dev)
-    return out
+    return res
 
 
 def large_field_leg(torch, P, L, lib, dev):
@@ -359,6 +373,8 @@ def generator_leg(torch, dev):
 
 # extra workloads that run alone with --leg (uniform signature)
 LEGS = {
+    "uneven_batch": lambda torch, P, L, lib, dev: uneven_leg(torch, P, L, lib, dev),
+    "speckle_chain": lambda torch, P, L, lib, dev: speckle_leg(torch, P, dev),
     "large_fields": lambda torch, P, L, lib, dev: large_field_leg(torch, P, L, lib, dev),
     "resample": lambda torch, P, L, lib, dev: resample_leg(torch, P, dev),
     "modes": lambda torch, P, L, lib, dev: modes_leg(torch, P, dev),

@@ -200,7 +200,9 @@ hipError_t launch_rows(const PassArgs<T>& a, bool inv, hipStream_t s) {
 
 // The pipelined class covers the row passes of strided complex64 batches without
 // spatial features (the passes of a plain propagation); anything else runs the plain
-// class of the same shape.
+// class of the same shape.  (Over an uneven batch's per-field pointer tables, read one
+// tile ahead, it measured slower than the plain class: 2.43 against 2.36 ms for
+// 2048 x 128^2 + 1536 x 256^2 + 512 x 512^2.)
 template <typename T, int N, int E, int LINES, int PS>
 hipError_t launch_rows_pipe(const PassArgs<T>& a, bool inv, hipStream_t s) {
   constexpr bool TILE_OK = LINES % tile_rows(N) == 0 && (N / E) % tile_cols(N) == 0;

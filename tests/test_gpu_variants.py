@@ -84,8 +84,13 @@ def test_every_variant_matches_oracle(restore_selection, n, dtype, which):
         lib.lp2d_set_variant(n, dt, kind, default)
 
 
+@pytest.mark.parametrize("kind", [0, 1], ids=["rows", "cols"])
 @pytest.mark.parametrize("dtype", ["c64", "c128"])
-def test_every_column_variant_on_uneven_tables(restore_selection, dtype):
+def test_every_variant_on_uneven_tables(restore_selection, dtype, kind):
+    """Every row and every column variant of 256 and 512 points over the per-field
+    pointer, distance and aperture tables of an uneven batch, with separate
+    destinations (a pipelined row variant runs its plain class of the same shape
+    there)."""
     from lightprop2d_amd import propagation as P
     lib = restore_selection
     dt = 0 if dtype == "c64" else 1
@@ -104,17 +109,17 @@ def test_every_column_variant_on_uneven_tables(restore_selection, dtype):
     dsts = [torch.empty_like(s) for s in srcs]
     ub = P.UnevenBatch(srcs, dsts, zs, area, wl, aperture_radius=aps)
     for n in (256, 512):
-        default = lib.lp2d_get_variant(n, dt, 1)
-        for v, name in enumerate(_names(lib, n, dt, 1)):
-            print(f"uneven {dtype} n={n} column variant={v} [{name}]", flush=True)
-            assert lib.lp2d_set_variant(n, dt, 1, v) == 0
+        default = lib.lp2d_get_variant(n, dt, kind)
+        for v, name in enumerate(_names(lib, n, dt, kind)):
+            print(f"uneven {dtype} n={n} kind={kind} variant={v} [{name}]", flush=True)
+            assert lib.lp2d_set_variant(n, dt, kind, v) == 0
             for d in dsts:
                 d.zero_()
             ub.run()
             for s, d, e in zip(sizes, dsts, exps):
                 if s == n:
                     assert rel_l2(d.cpu().numpy(), e) < tol, name
-        lib.lp2d_set_variant(n, dt, 1, default)
+        lib.lp2d_set_variant(n, dt, kind, default)
 
 
 @pytest.mark.parametrize("n,batch", [(128, 8300), (512, 300), (1024, 72), (256, 200), (256, 1)])
