@@ -371,6 +371,25 @@ def test_full_size_round_trip_4096():
     assert rel_l2(host(w), u) < 2e-6
 
 
+def test_4096_batch_of_6_in_column_groups_of_4():
+    """6 fields of 4096^2 c64: the column pass takes groups of at most 4 fields at this
+    length (a full group of 4 and a partial one of 2) over the 2 x 4 block-tiled plane;
+    a field of each group against the oracle, every field through the round trip."""
+    from lightprop2d_amd import propagation as P
+    n, area, wl, z, na = 4096, 2e-1, 800e-7, 0.2, 0.05
+    B = 6
+    rs = np.random.RandomState(6)
+    u = cplx(rs, (B, n, n), np.complex64)
+    tu = to_dev(u)
+    v = P.propagate(tu, z, area, wl, aperture_radius=na / wl)
+    for b in (1, 5):
+        print(f"n=4096 batch=6 field={b}", flush=True)
+        exp = oracle.propagate_field(u[b], area, wl, z, aperture_radius=na / wl)
+        assert rel_l2(host(v[b]), exp) < 1e-5
+    w = P.propagate(P.propagate(tu, z, area, wl), -z, area, wl)
+    assert rel_l2(host(w), u) < 2e-6
+
+
 # ------------------------------------------------------------------ diagnostics (beam2d.py:669-752)
 @pytest.mark.parametrize("bits", [64, 128])
 def test_diagnostics_vs_oracle(bits):
