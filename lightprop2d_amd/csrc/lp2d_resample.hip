@@ -181,12 +181,28 @@ __global__ __launch_bounds__(kInterpThreads) void k_interp_linear(
   const double2 cx = chirp ? chirp_factor(d, j) : make_double2(1.0, 0.0);
   const C* win = in + ((long long)b * n + d.win_off) * n + d.win_off + tx.l;
   C* dst = out + ((long long)b * n_out + i0) * n_out + j;
-#pragma unroll 4
+  // Source samples of the current row pair, c<x tap><y tap>.  Consecutive output rows
+  // mostly fall in the same or the next source interval (zoom, magnification < 2), so
+  // the samples are reused: the row taps are the same in every lane, so the branches
+  // are uniform.
+  C c00, c01, c10, c11;
+  int lprev = -2;
   for (int r = 0; r < rows; ++r) {
     const AxisTap ty = ty_s[r];
-    const C* r0 = win + (long long)ty.l * n;
-    const C* r1 = r0 + n;
-    const C c00 = r0[0], c01 = r1[0], c10 = r0[1], c11 = r1[1];  // c<x tap><y tap>
+    if (ty.l != lprev) {
+      const C* r1 = win + (long long)(ty.l + 1) * n;
+      if (ty.l == lprev + 1) {
+        c00 = c01;
+        c10 = c11;
+      } else {
+        const C* r0 = r1 - n;
+        c00 = r0[0];
+        c10 = r0[1];
+      }
+      c01 = r1[0];
+      c11 = r1[1];
+      lprev = ty.l;
+    }
     double vr = ((double)c00.x * tx.h0) * ty.h0;
     double vi = ((double)c00.y * tx.h0) * ty.h0;
     vr += ((double)c01.x * tx.h0) * ty.h1;
