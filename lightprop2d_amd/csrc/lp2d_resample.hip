@@ -117,7 +117,7 @@ __global__ void k_coarse2(const float2* __restrict__ in, float2* __restrict__ ou
 // [t_0, t_{len-1}], the interval is the last l <= len-2 with t_l <= q, and fpbspl's
 // basis values are f*(t_{l+1} - q) and f*(q - t_l) with f = 1/(t_{l+1} - t_l).  The
 // knots t_k and queries q_j are formed as integer index times spacing, the way the
-// reference builds X (beam2d.py:302-304), so they carry the reference's rounding.
+// reference builds X (beam2d.py:300-301), so they carry the reference's rounding.
 struct AxisTap {
   int l;          // window sample of the left tap
   double h0, h1;  // weights of samples l and l + 1

@@ -325,7 +325,7 @@ def coarse(field: torch.Tensor, mean_order: int) -> torch.Tensor:
 def interp_linear(field: torch.Tensor, n_out: int, desc: "L.lp2d_interp_desc") -> torch.Tensor:
     """Bilinear resampling (..., n, n) -> (..., n_out, n_out): scipy's
     interp2d(kind='linear') on the regular grid ``desc`` describes, with its output
-    scale and chirp (the interp2d calls of beam2d.py:426-434, 525-535).  Computed in
+    scale and chirp (the interp2d calls of beam2d.py:428-434, 527-535).  Computed in
     FP64 and rounded once to the field's dtype."""
     lib = _lib()
     n, batch, dt = _field_geom(field)
@@ -338,14 +338,14 @@ def interp_linear(field: torch.Tensor, n_out: int, desc: "L.lp2d_interp_desc") -
 
 
 def crop_desc(n: int, area_size: float, area_new: float, npoints: int = 0):
-    """Window and grids of Beam2D.crop (beam2d.py:419-432): the centred samples with
+    """Window and grids of Beam2D.crop (beam2d.py:419-434): the centred samples with
     |X| < area_new / 2 of the n-point grid of side area_size (an even count 2h around
     n // 2), taken at the new grid X' = arange(-m // 2, m // 2) * area_new / m
     (m = npoints, or n when 0).  Returns (desc, m)."""
     n = int(n)
     m = int(npoints) if npoints else n
     dL = area_size / n
-    X = np.arange(-n // 2, n // 2, 1) * dL            # beam2d.py:302-303
+    X = np.arange(-n // 2, n // 2, 1) * dL            # beam2d.py:300-301
     h = len(X[np.abs(X) < area_new / 2]) // 2
     if h < 1:
         raise ValueError(f"crop to {area_new} keeps {2 * h} samples per axis; interp2d "
@@ -368,7 +368,7 @@ def crop(field: torch.Tensor, area_size: float, area_new: float, npoints: int = 
 
 
 def lens_image_desc(n: int, area_size: float, wl: float, l1: float, l2: float):
-    """Grids of Beam2D.lens_image (beam2d.py:524-535): the field on X = arange(-n // 2,
+    """Grids of Beam2D.lens_image (beam2d.py:525-535): the field on X = arange(-n // 2,
     n // 2) * dL taken at sorted(X * s), s = -l1 / l2, times s and the chirp
     exp(-i k0 (l1 + l2) / (2 l2^2) (X^2 + Y^2)), k0 = 1 / wl."""
     n = int(n)

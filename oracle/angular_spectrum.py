@@ -142,7 +142,7 @@ def _fitpack_linear_taps(t, q):
 def interp2d_linear(x, y, z, xq, yq):
     """``scipy.interpolate.interp2d(x, y, z, kind='linear')(xq, yq)`` for data on a
     rectangular grid, as it ran before scipy 1.14 removed it (the calls of Beam2D.crop
-    and Beam2D.lens_image, beam2d.py:426-427, 431, 525-526, 532).
+    and Beam2D.lens_image, beam2d.py:428-429, 434, 527-528, 534).
 
     interp2d fitted FITPACK's regrid_smth with kx = ky = 1 and s = 0 -- the piecewise
     bilinear interpolant with a breakpoint at every sample -- and evaluated it with
@@ -296,7 +296,7 @@ class OracleBeam2D:
         self.spectrum = fft2(self.field)
 
     def lens_image(self, f, l1, l2):
-        # beam2d.py:524-536 (f is unused there too).  The reference's numpy path scales
+        # beam2d.py:525-536 (f is unused there too).  The reference's numpy path scales
         # self.X in place through the alias X = self._np(self.X); its cupy path (.get())
         # copies.  This follows the copy, leaving the grids alone.
         scale_factor = - l1 / l2

@@ -161,8 +161,8 @@ def test_argument_errors_of_the_interpolation_entry_point(lib):
 
 def test_interpolation_grids_are_the_reference_coordinates():
     """The descriptors of crop / lens_image reproduce the reference's coordinate arrays
-    bit for bit: the window's knots old_X and the new grid X (beam2d.py:419-432), and
-    the sorted queries X * (-l1/l2) (beam2d.py:524-532)."""
+    bit for bit: the window's knots old_X and the new grid X (beam2d.py:419-434), and
+    the sorted queries X * (-l1/l2) (beam2d.py:525-534)."""
     import numpy as np
     from lightprop2d_amd import propagation as P
 
