@@ -464,6 +464,29 @@ def test_uneven_batch_per_field_optics(dt):
         assert rel_l2(host(d), exp) < tol(dt)
 
 
+def test_uneven_batch_of_4096_fields():
+    """Five 4096^2 complex64 fields and one 2048^2 in an uneven batch: the 4096 bucket
+    runs the column pass over per-field pointer and optics tables on the 2 x 4
+    block-tiled plane, in groups of at most 4 fields (4 + 1).  Per-field z and
+    apertures; two 4096 fields (one per group) and the 2048 field against the oracle."""
+    from lightprop2d_amd import propagation as P
+    rs = np.random.RandomState(9)
+    area, wl = 2e-1, 800e-7
+    sizes = [4096, 4096, 2048, 4096, 4096, 4096]
+    zs = [0.2, 0.1, 0.15, 0.05, 0.3, 0.25]
+    aps = [0.05 / wl, 0.0, 0.04 / wl, 0.03 / wl, 0.0, 0.06 / wl]
+    host_in = [cplx(rs, (n, n), np.complex64) for n in sizes]
+    srcs = [to_dev(f) for f in host_in]
+    dsts = [torch.empty_like(t) for t in srcs]
+    P.UnevenBatch(srcs, dsts, zs, area, wl, aperture_radius=aps).run()
+    torch.cuda.synchronize()
+    for i in (0, 2, 5):
+        print(f"uneven 4096 batch field={i} n={sizes[i]}", flush=True)
+        exp = oracle.propagate_field(host_in[i], area, wl, zs[i],
+                                     aperture_radius=aps[i] or None)
+        assert rel_l2(host(dsts[i]), exp) < 1e-5
+
+
 @pytest.mark.parametrize("dt,tol", [(np.complex64, 1e-5), (np.complex128, 1e-10)],
                          ids=["c64", "c128"])
 def test_maximum_size_8192(dt, tol):
