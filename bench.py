@@ -16,6 +16,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -106,19 +107,31 @@ def cpu_baseline(n, budget_s, workers=None):
             "omp_num_threads": os.environ.get("OMP_NUM_THREADS", "")}
 
 
-def profiled_traffic(kernel_name):
-    """HBM bytes per launch of `kernel_name` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (profiles/hbm_traffic_propagate_1024_c64.md), or None."""
-    p = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                     "hbm_traffic_propagate_1024_c64.md")
+TRAFFIC_FILE = "hbm_traffic_propagate_1024_c64.md"
+
+
+def profiled_traffic(n, batch):
+    """HBM bytes per launch of each sweep of the forward step, keyed by the tracer's
+    kernel names, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (profiles/hbm_traffic_propagate_1024_c64.md, lines
+    ``sweep_traffic_bytes[<name>]: <bytes>``), or {} for another workload."""
+    p = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", TRAFFIC_FILE)
+    out = {}
     try:
-        kv = dict(l.split(": ", 1) for l in open(p).read().splitlines() if ": " in l
-                  and l.split(": ", 1)[0] in ("bench_kernel", "traffic_bytes_per_launch"))
+        text = open(p).read()
     except OSError:
-        return None
-    if kv.get("bench_kernel") != kernel_name:
-        return None
-    return int(kv["traffic_bytes_per_launch"])
+        return out
+    kv = {}
+    for line in text.splitlines():
+        m = re.match(r"^sweep_traffic_bytes\[(.+)\]: (\d+)$", line.strip())
+        if m:
+            out[m.group(1)] = int(m.group(2))
+        elif ": " in line:
+            k, v = line.split(": ", 1)
+            kv[k.strip()] = v.strip()
+    if kv.get("workload_n") != str(n) or kv.get("workload_batch") != str(batch):
+        return {}
+    return out
 
 
 def trace_stats(lib, L):
@@ -420,23 +433,30 @@ def main():
         P.propagate_adjoint(gout, u, z, area, wl, phase=phi, out=gin)
 
     def timed(step, K, W):
-        """Wall time of K steps (untraced), then per-kernel times from a traced pass of
-        min(K, 10) more steps: HIP events around every launch perturb back-to-back
-        kernels, so they stay out of the measurement of `value`."""
+        """K steps after W warm-up ones: (wall seconds, GPU seconds between two HIP
+        events recorded on the launch stream around the K steps, per-kernel times).
+        The per-kernel times come from a separate traced pass of min(K, 10) steps:
+        events around every launch perturb back-to-back kernels, so they stay out of
+        the timed region."""
         for _ in range(W):
             step()
         torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
         t0 = time.perf_counter()
+        e0.record()  # torch's current stream: the stream every lp2d call launches on
         for _ in range(K):
             step()
+        e1.record()
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
+        gpu = e0.elapsed_time(e1) / 1e3
         L.check(lib.lp2d_trace_enable(1))
         for _ in range(min(K, 10)):
             step()
         torch.cuda.synchronize()
         L.check(lib.lp2d_trace_enable(0))
-        return el, trace_stats(lib, L)
+        return el, gpu, trace_stats(lib, L)
 
     # plain device copy of the same bytes: achievable-bandwidth reference, and the
     # calibration dispatch for rocprof's FETCH_SIZE / WRITE_SIZE counters
@@ -452,28 +472,53 @@ def main():
     copy_s = ev0.elapsed_time(ev1) / 1e3 / 10
     copy_gbs = 2 * u.numel() * 8 / copy_s / 1e9
 
-    el, kt = timed(fwd_step, args.steps, args.warmup)
+    el, gpu_s, kt = timed(fwd_step, args.steps, args.warmup)
     props = B * args.steps
     value = props / el
-    el2, kt2 = timed(fwd_adj_step, max(1, args.steps // 2), 2)
-    fa_rate = B * max(1, args.steps // 2) / el2
+    K2 = max(1, args.steps // 2)
+    el2, gpu2_s, kt2 = timed(fwd_adj_step, K2, 2)
+    fa_rate = B * K2 / el2
 
-    # dominant kernel of the forward step: algorithmic bytes per launch = 16 n^2 per
-    # field (each pass reads and writes the field once) x B fields
-    dom = max(kt.items(), key=lambda kv: kv[1][1])
-    dom_name, (dom_launches, dom_ms) = dom
-    avg_s = dom_ms / 1e3 / dom_launches
-    alg_bytes = 16.0 * n * n * B
-    achieved = alg_bytes / avg_s / 1e9
-    e2e = 16.0 * n * n * value / 1e9
-    traffic = profiled_traffic(dom_name) if B == 32 else None
+    # The unit of work is one field-propagation, spatial field in HBM -> spatial field
+    # in HBM: B_alg = 16 n^2 bytes (the field read once and written once, complex64).
+    # One step launches the three sweeps (forward rows, columns with H, inverse rows)
+    # over the B fields, so the step's algorithmic bytes are 16 n^2 B, and the
+    # roofline fraction is that over the step's GPU time (HIP events around the timed
+    # steps).  Each sweep also reads and writes every field once, so a sweep alone is
+    # credited with the same bytes in `sweeps` (its own rate, not the step's).
+    alg_step = 16.0 * n * n * B
+    achieved = alg_step * args.steps / gpu_s / 1e9
+    wall_gbs = 16.0 * n * n * value / 1e9
+    traffic = profiled_traffic(n, B)
+    sweeps = {}
+    for name, (c, ms) in kt.items():
+        avg = ms / 1e3 / c
+        sw = {"avg_us": round(avg * 1e6, 2), "alg_bytes": int(alg_step),
+              "GBs": round(alg_step / avg / 1e9, 1),
+              "frac": round(alg_step / avg / 1e9 / HBM_PEAK_GBS, 4)}
+        if name in traffic:
+            sw["traffic"] = traffic[name]
+            sw["traffic_ratio"] = round(traffic[name] / alg_step, 3)
+        sweeps[name] = sw
+    step_traffic = (sum(traffic[k] for k in kt) if traffic and all(k in traffic for k in kt)
+                    else None)
+    kernel_sum_s = sum(ms / 1e3 / c for c, ms in kt.values())
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "traffic_source": "profiles/hbm_traffic_propagate_1024_c64.md" if traffic else None,
-                "kernel": dom_name, "kernel_avg_us": round(avg_s * 1e6, 2),
-                "alg_bytes_per_launch": int(alg_bytes),
-                "end_to_end_GBs": round(e2e, 1),
-                "end_to_end_frac": round(e2e / HBM_PEAK_GBS, 4),
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": step_traffic,
+                "traffic_ratio": round(step_traffic / alg_step, 3) if step_traffic else None,
+                "traffic_per_propagation": round(step_traffic / B) if step_traffic else None,
+                "traffic_source": ("profiles/" + TRAFFIC_FILE) if step_traffic else None,
+                "launch": (f"one step: the {len(kt)} sweeps of a field-to-field "
+                           f"propagation over {B} fields of {n}^2 complex64"),
+                "alg_bytes_per_launch": int(alg_step),
+                "alg_bytes_per_propagation": int(16 * n * n),
+                "step_gpu_us": round(gpu_s / args.steps * 1e6, 2),
+                "kernel_sum_us": round(kernel_sum_s * 1e6, 2),
+                "kernel_sum_frac": round(alg_step / kernel_sum_s / 1e9 / HBM_PEAK_GBS, 4),
+                "wall_GBs": round(wall_gbs, 1),
+                "wall_frac": round(wall_gbs / HBM_PEAK_GBS, 4),
+                "sweeps": sweeps,
                 "copy_GBs": round(copy_gbs, 1)}
     kernels = {k: {"launches": c, "avg_us": round(ms * 1e3 / c, 2)} for k, (c, ms) in kt.items()}
 
@@ -503,7 +548,8 @@ def main():
                                f"wl 632 nm, z 50 um, inputs resident in HBM",
                    "n": n, "batch": B, "parallelism": "single GPU"},
         "fwd_adjoint": {"value": round(fa_rate, 1), "unit": "field-propagations/s",
-                        "ms_per_step": round(el2 / max(1, args.steps // 2) * 1e3, 4),
+                        "ms_per_step": round(el2 / K2 * 1e3, 4),
+                        "gpu_ms_per_step": round(gpu2_s / K2 * 1e3, 4),
                         "ratio_to_fwd": round(value / fa_rate, 3),
                         "workload": "forward with phase screen + adjoint with phase gradient"},
         "roofline": roofline,

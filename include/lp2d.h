@@ -76,8 +76,10 @@ typedef struct lp2d_optics {
 } lp2d_optics;
 
 /* ---------------------------------------------------------------- library */
-/* Prepares the twiddle tables on `device` (idempotent, thread-safe).  Every other
- * entry point calls it for the current device on first use. */
+/* Prepares the twiddle tables on `device` (idempotent, thread-safe; allocates).
+ * Every other entry point calls it for the current device on first use; a first use
+ * on a stream that is being captured into a graph returns LP2D_E_INIT instead, so
+ * callers that capture call lp2d_init beforehand. */
 int lp2d_init(int device);
 const char* lp2d_last_error(void);
 int lp2d_version(void);

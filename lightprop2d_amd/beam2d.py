@@ -9,6 +9,20 @@ Differences, all deliberate:
   ``use_gpu`` flag is accepted and ignored (the HIP path always runs);
 * ``complex_bits=64`` computes in complex64 kernels, 128 in complex128 kernels;
 * ``init_field`` is copied to the device rather than aliased;
+* ``field`` is materialised lazily: ``propagate`` and ``spectral_filter`` only multiply
+  the carried ``spectrum`` (one elementwise sweep, the transfer function of a
+  ``propagate`` is folded into the next spectral operation), and the inverse FFT runs
+  when ``field`` (or ``iprofile``, ``centroid``, ...) is next read.  Every value a
+  caller can observe is the reference's: reading ``spectrum`` also freezes the field
+  it stood for, and assigning either attribute keeps the other as the reference
+  would.  Assigned arrays are stored in the beam's complex dtype;
+* ``f_gen`` callables receive the device grids (``X``/``Y`` or ``Kx``/``Ky``) as torch
+  tensors; one written for numpy arrays (``np.exp``, ``np.abs`` ...) that raises on
+  them is called again on host copies of the grids, and its result is uploaded (any
+  in-place shift of its grid arguments is copied back, as the reference's numpy path
+  would keep it);
+* assigning ``Kz`` replaces the transfer function's Kz as in the reference (propagate
+  then multiplies by exp(-2*pi*i*frac(Kz*z)) built from the assigned grid in FP64);
 * ``npoints`` is a power of two from 2 to 8192 (the device FFT); ``expand``,
   ``coarse`` and ``crop`` check the new grid and raise ``Lp2dError`` before they
   change any state;
@@ -83,23 +97,99 @@ class Beam2D:
             self.complex = torch.complex64
         elif self.complex_bits == 128:
             self.complex = torch.complex128
+        self._field = None
+        self._spectrum = None
+        self._pending_z = 0.0
         self._construct_grids()
         if self.init_field_gen is not None:
-            self.field = self._asxp(
-                self.init_field_gen(self.X, self.Y, *self.init_gen_args))
+            field = self._asxp(self._call_gen(self.init_field_gen, self.X, self.Y,
+                                              self.init_gen_args))
         elif self.init_field is not None:
-            self.field = self._asxp(self.init_field)
+            field = self._asxp(self.init_field)
         else:
             raise ValueError(
                 "Init field data is None : " +
                 "'init_field_gen' must be a function or 'init_field' must be an array.")
-        if self.field.dtype != self.complex:
-            self.field = self.field.to(self.complex)
-        self.field = self.field.contiguous()
+        self._field = self._field_tensor(field)
         if self.init_spectrum is None:
-            self.spectrum = self._fft2(self.field)
+            self._spectrum = self._fft2(self._field)
         else:
-            self.spectrum = self._asxp(self.init_spectrum).to(self.complex).contiguous()
+            self._spectrum = self._field_tensor(self._asxp(self.init_spectrum))
+
+    # ------------------------------------------------------------- lazy state
+    # Invariant: _spectrum is the reference's spectrum once the pending distance
+    # _pending_z (a propagate not yet applied) is multiplied in; _field is the
+    # reference's field, or None when it is ifft2 of that spectrum.
+    def _flush(self, masks=(), field_out=None) -> None:
+        """spectrum *= H(pending z) * masks in place (one elementwise sweep, or the
+        column pass of an inverse FFT into field_out)."""
+        z, self._pending_z = self._pending_z, 0.0
+        masks = list(masks)
+        if z != 0 and self._Kz_user:
+            masks.insert(0, self._user_transfer(z))
+            z = 0.0
+        if len(masks) > 2:  # the kernels take two multipliers per call
+            P.spectral_apply(self._spectrum, masks[:1], z=z, area_size=self.area_size,
+                             wl=self.wl, evanescent=self.evanescent)
+            masks, z = masks[1:], 0.0
+        if z == 0 and not masks:
+            if field_out is not None:
+                P.ifft2(self._spectrum, out=field_out)
+            return
+        P.spectral_apply(self._spectrum, masks, z=z, area_size=self.area_size, wl=self.wl,
+                         evanescent=self.evanescent, field_out=field_out)
+
+    def _field_now(self) -> torch.Tensor:
+        """The current field, computed from the spectrum if it is stale."""
+        if self._field is None:
+            out = torch.empty_like(self._spectrum)
+            self._flush(field_out=out)
+            self._field = out
+        return self._field
+
+    def _spectrum_now(self) -> torch.Tensor:
+        """The current spectrum with any pending propagation applied."""
+        if self._pending_z != 0:
+            self._flush()
+        return self._spectrum
+
+    @property
+    def field(self) -> torch.Tensor:
+        return self._field_now()
+
+    @field.setter
+    def field(self, value):
+        # the reference rebinds field and leaves spectrum as it was
+        self._spectrum_now()
+        self._field = self._field_tensor(self._asxp(value))
+
+    @property
+    def spectrum(self) -> torch.Tensor:
+        # handed out, the spectrum may be changed in place by the caller; the field of
+        # this state is the reference's ifft2 of it *now*, so it is fixed first
+        self._field_now()
+        return self._spectrum
+
+    @spectrum.setter
+    def spectrum(self, value):
+        self._field_now()
+        self._spectrum = self._field_tensor(self._asxp(value))
+
+    def _call_gen(self, f, a: torch.Tensor, b: torch.Tensor, args):
+        """f(a, b, *args) on the device grids, or, when f fails on torch tensors (numpy
+        code), on host copies of them, with the result uploaded.  a / b are the grid
+        vectors (b a view of a); an in-place shift of them by f is kept in both cases,
+        as the reference's grids keep it (profiles.py:69-70)."""
+        saved = a.clone()
+        try:
+            return f(a, b, *args)
+        except (TypeError, RuntimeError, AttributeError):
+            a.copy_(saved)  # undo a partial in-place shift before the host call
+        ah = a.detach().cpu().numpy().copy()
+        bh = ah.reshape(tuple(b.shape))
+        r = f(ah, bh, *args)
+        a.copy_(torch.from_numpy(ah))
+        return r
 
     # ------------------------------------------------------------- array plumbing
     def _np(self, data) -> np.ndarray:
@@ -136,11 +226,12 @@ class Beam2D:
     def _update_obj(self, field: ArrayLike, spectrum: ArrayLike = None):
         """beam2d.py:283-288: replace the state without (or with) an FFT; z = 0."""
         self.z = 0
-        self.field = self._field_tensor(self._asxp(field))
+        self._pending_z = 0.0
+        self._field = self._field_tensor(self._asxp(field))
         if spectrum is None:
-            self.spectrum = self._fft2(self.field)
+            self._spectrum = self._fft2(self._field)
         else:
-            self.spectrum = self._field_tensor(self._asxp(spectrum))
+            self._spectrum = self._field_tensor(self._asxp(spectrum))
 
     def _construct_grids(self):
         """beam2d.py:290-308.  Kz comes from the same kernel code that generates it on
@@ -153,6 +244,7 @@ class Beam2D:
         self.Kx = self._k_grid(self.dL, self.npoints)
         self.Ky = self.Kx.reshape((-1, 1))
         self._Kz = None
+        self._Kz_user = False
 
     @property
     def Kz(self) -> torch.Tensor:
@@ -162,7 +254,17 @@ class Beam2D:
 
     @Kz.setter
     def Kz(self, value):
-        self._Kz = value
+        """An assigned Kz replaces the on-chip transfer function in propagate (the
+        reference's propagate reads self.Kz, beam2d.py:485)."""
+        self._spectrum_now()
+        self._Kz = self._asxp(value).to(torch.float64)
+        self._Kz_user = True
+
+    def _user_transfer(self, z: float) -> torch.Tensor:
+        """exp(-2*pi*i*frac(Kz*z)) from the assigned Kz, FP64 (beam2d.py:485-489)."""
+        d = self._Kz * z
+        ph = 2 * np.pi * (torch.trunc(d) - d)
+        return torch.complex(torch.cos(ph), torch.sin(ph))
 
     # ------------------------------------------------------------- operators
     def coordinate_filter(self, f_init: ArrayLike = None, f_gen=None, fargs: tuple = ()) -> None:
@@ -171,25 +273,30 @@ class Beam2D:
         if f_gen is not None:
             assert callable(f_gen)
             assert isinstance(fargs, Iterable)
-            masks.append(f_gen(self.X, self.Y, *fargs))
+            masks.append(self._call_gen(f_gen, self.X, self.Y, fargs))
         if f_init is not None:
             assert isinstance(f_init, (np.ndarray, torch.Tensor))
             masks.append(f_init)
-        self.spectrum = torch.empty_like(self.field)
-        P.spatial_apply(self.field, masks, spectrum_out=self.spectrum)
+        field = self._field_now()
+        spec = torch.empty_like(field)
+        P.spatial_apply(field, masks, spectrum_out=spec)
+        self._spectrum = spec
 
     def spectral_filter(self, f_init: ArrayLike = None, f_gen=None, fargs: tuple = ()) -> None:
-        """beam2d.py:344-373: spectrum *= f_gen(Kx, Ky, *fargs) * f_init; field = ifft2."""
+        """beam2d.py:344-373: spectrum *= f_gen(Kx, Ky, *fargs) * f_init; field = ifft2.
+
+        The multiply (with a pending propagate's transfer function) runs now; the
+        inverse FFT when the field is next read."""
         masks = []
         if f_gen is not None:
             assert callable(f_gen)
             assert isinstance(fargs, Iterable)
-            masks.append(f_gen(self.Kx, self.Ky, *fargs))
+            masks.append(self._call_gen(f_gen, self.Kx, self.Ky, fargs))
         if f_init is not None:
             assert isinstance(f_init, (np.ndarray, torch.Tensor))
             masks.append(f_init)
-        self.field = torch.empty_like(self.spectrum)
-        P.spectral_apply(self.spectrum, masks, field_out=self.field)
+        self._flush(masks)
+        self._field = None
 
     def expand(self, area_size: float):
         """beam2d.py:375-404: enlarge the area at constant dL, replicating the edge rows
@@ -197,44 +304,46 @@ class Beam2D:
         npoints = int(self.npoints * area_size / self.area_size)
         npoints -= npoints % 2
         _check_grid(npoints)
-        field = P.expand(self.field, npoints)
+        field = P.expand(self._field_now(), npoints)
         self.npoints = npoints
         self.area_size = area_size
         self._construct_grids()
-        self.field = field
-        self.spectrum = self._fft2(self.field)
+        self._set_state(field)
 
     def crop(self, area_size: float, npoints: int = 0):
         """beam2d.py:406-435: keep the centred samples with |X| < area_size/2 and
         interpolate them bilinearly onto npoints (kept when 0) samples of the new area;
         spectrum = fft2(field)."""
         _check_grid(npoints or self.npoints)
-        field = P.crop(self.field, self.area_size, area_size, npoints)
+        field = P.crop(self._field_now(), self.area_size, area_size, npoints)
         if npoints:
             self.npoints = npoints
         self.area_size = area_size
         self._construct_grids()
-        self.field = field
-        self.spectrum = self._fft2(self.field)
+        self._set_state(field)
 
     def lens_image(self, f: float, l1: float, l2: float):
         """beam2d.py:513-536: the image between conjugate planes l1 before and l2 after
         the lens -- the field resampled at X * (-l1/l2) (sorted, so never flipped),
         times -l1/l2 and the chirp exp(-i k0 (l1 + l2) r^2 / (2 l2^2)); f is unused,
         as in the reference.  spectrum = fft2(field)."""
-        self.field = P.lens_image(self.field, self.area_size, self.wl, l1, l2)
-        self.spectrum = self._fft2(self.field)
+        self._set_state(P.lens_image(self._field_now(), self.area_size, self.wl, l1, l2))
 
     def coarse(self, mean_order: int = 1):
         """beam2d.py:437-460: block-average the field over mean_order x mean_order
         (npoints //= mean_order); spectrum = fft2(field)."""
         if mean_order >= 1 and self.npoints % mean_order == 0:
             _check_grid(self.npoints // mean_order)
-        field = P.coarse(self.field, mean_order)     # ValueError for a non-divisor
+        field = P.coarse(self._field_now(), mean_order)  # ValueError for a non-divisor
         self.npoints //= mean_order
         self._construct_grids()
-        self.field = field
-        self.spectrum = self._fft2(self.field)
+        self._set_state(field)
+
+    def _set_state(self, field: torch.Tensor) -> None:
+        """field = field; spectrum = fft2(field) (a new grid or a resampled field)."""
+        self._pending_z = 0.0
+        self._field = field
+        self._spectrum = self._fft2(field)
 
     # ------------------------------------------------------------- mode bases
     def _expand_basis(self, modes_list) -> torch.Tensor:
@@ -248,52 +357,56 @@ class Beam2D:
         the central window of the field (HIP kernel); 'lstsq' = least squares of
         modes^T c = field (torch.linalg on the device)."""
         if mode == "scalar":
-            self.modes_coeffs = P.modes_project(self.field, modes_list)
+            self.modes_coeffs = P.modes_project(self._field_now(), modes_list)
         elif mode == "lstsq":
             m = P.mode_matrix(modes_list, torch.complex128, self._device)
             nb = int(round(m.shape[1] ** 0.5))
             d = (self.npoints - nb) // 2
-            f = self.field[d:self.npoints - d, d:self.npoints - d].reshape(-1, 1)
+            f = self._field_now()[d:self.npoints - d, d:self.npoints - d].reshape(-1, 1)
             self.modes_coeffs = torch.linalg.lstsq(m.T, f.to(torch.complex128)).solution[:, 0]
         return self.modes_coeffs
 
     def fast_deconstruct_by_modes(self, modes_matrix_t, modes_matrix_dot_t) -> torch.Tensor:
         """beam2d.py:605-651: lstsq(Gram matrix, modes_matrix_t . field), the product by
         the projection kernel, the small solve with torch.linalg on the device."""
-        b = P.modes_project(self.field, modes_matrix_t)
+        b = P.modes_project(self._field_now(), modes_matrix_t)
         g = self._asxp(modes_matrix_dot_t).to(torch.complex128)
         self.modes_coeffs = torch.linalg.lstsq(g, b.reshape(-1, 1)).solution[:, 0]
         return self.modes_coeffs
 
     def construct_by_modes(self, modes_list, modes_coeffs):
         """beam2d.py:653-667: field = sum_i c_i M_i (zero-padded); spectrum = fft2."""
-        self.field = P.modes_construct(modes_list, modes_coeffs, self.npoints, self.complex,
-                                       self._device)
-        self.spectrum = self._fft2(self.field)
+        self._set_state(P.modes_construct(modes_list, modes_coeffs, self.npoints, self.complex,
+                                          self._device))
 
     def propagate(self, z: float):
         """beam2d.py:462-490: spectrum *= exp(-2*pi*i*frac(Kz*z)); field = ifft2(spectrum).
 
         The transfer function is generated on chip (FP64 Kz*z reduced to its fractional
-        cycle); evanescent modes pass unchanged, as in the reference."""
+        cycle); evanescent modes pass unchanged, as in the reference.  The multiply is
+        folded into the next operation on the spectrum (a spectral filter, the next
+        propagate, or the inverse FFT when the field is read)."""
         if z == 0:
             return
         self.z += z
-        self.field = torch.empty_like(self.spectrum)
-        P.spectral_apply(self.spectrum, (), z=z, area_size=self.area_size, wl=self.wl,
-                         evanescent=self.evanescent, field_out=self.field)
+        if self._pending_z != 0:
+            self._flush()
+        self._pending_z = z
+        self._field = None
 
     def lens(self, f: float):
         """beam2d.py:492-511: field *= exp(i*(X^2+Y^2)*k0/(2f)); spectrum = fft2(field)."""
-        self.spectrum = torch.empty_like(self.field)
-        P.spatial_apply(self.field, (), lens_f=f, area_size=self.area_size, wl=self.wl,
-                        spectrum_out=self.spectrum)
+        field = self._field_now()
+        spec = torch.empty_like(field)
+        P.spatial_apply(field, (), lens_f=f, area_size=self.area_size, wl=self.wl,
+                        spectrum_out=spec)
+        self._spectrum = spec
 
     # ------------------------------------------------------------- observables
     @property
     def iprofile(self):
         """beam2d.py:706-721: |field|^2 (host numpy when numpy_output)."""
-        profile = P.intensity(self.field)
+        profile = P.intensity(self._field_now())
         if self.numpy_output:
             return self._np(profile)
         return profile
@@ -301,7 +414,7 @@ class Beam2D:
     @property
     def phiprofile(self):
         """beam2d.py:723-738: arg(field)."""
-        profile = torch.angle(self.field)
+        profile = torch.angle(self._field_now())
         if self.numpy_output:
             return self._np(profile)
         return profile
@@ -311,21 +424,21 @@ class Beam2D:
         """beam2d.py:669-685: intensity-weighted centroid (Xc, Yc, nxc, nyc)."""
         from .diagnostics import moments
         n = self.npoints // 2
-        Xc, Yc, _, _ = moments(self.field, self.dL)
+        Xc, Yc, _, _ = moments(self._field_now(), self.dL)
         return Xc, Yc, int(Xc / self.dL) + n, int(Yc / self.dL) + n
 
     @property
     def D4sigma(self) -> Tuple[float, float]:
         """beam2d.py:687-704: D = 4*sigma along x and y."""
         from .diagnostics import moments
-        _, _, sx, sy = moments(self.field, self.dL)
+        _, _, sx, sy = moments(self._field_now(), self.dL)
         return 4 * sx, 4 * sy
 
     @property
     def centroid_intensity(self) -> float:
         """beam2d.py:740-752: |A|^2 at the centroid pixel."""
         _, _, nxc, nyc = self.centroid
-        v = self.field[nyc, nxc]
+        v = self._field_now()[nyc, nxc]
         return float((v.real ** 2 + v.imag ** 2).item())
 
     def __repr__(self) -> str:

@@ -125,12 +125,21 @@ int init_device(int device) {
   return rc;
 }
 
-int current_tables(DeviceTables** out) {
+// The current device's tables, initialising them on first use.  Initialisation
+// allocates and synchronises, which a stream under graph capture does not allow: a
+// first call on such a stream fails with LP2D_E_INIT instead (lp2d_init beforehand).
+int current_tables(DeviceTables** out, hipStream_t s) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
   if (dev < 0 || dev >= kMaxDevices) return fail(LP2D_E_ARG, "device %d out of range", dev);
   if (!g_dev[dev].ready) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (s && hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone)
+      return fail(LP2D_E_INIT,
+                  "liblp2d is not initialised on device %d and the stream is being captured: "
+                  "call lp2d_init(%d) before graph capture", dev, dev);
+    (void)hipGetLastError();
     int rc = init_device(dev);
     if (rc) return rc;
   }
@@ -476,7 +485,7 @@ unsigned grid_for(long long total) {
 template <typename T>
 int op_fft2(int n, int lg, int batch, const void* in, void* out, int inverse, hipStream_t s) {
   DeviceTables* t;
-  if (int rc = current_tables(&t)) return rc;
+  if (int rc = current_tables(&t, s)) return rc;
   PassArgs<T> a = base_args<T>(t, n, batch);
   if (use_small<T>(lg)) {
     a.src = (const cx<T>*)in;
@@ -508,7 +517,7 @@ int op_spatial(int n, int lg, int batch, void* field, const lp2d_mask* masks, in
                const lp2d_optics* o, const float* phase, int64_t phase_sb, void* spec,
                hipStream_t s) {
   DeviceTables* t;
-  if (int rc = current_tables(&t)) return rc;
+  if (int rc = current_tables(&t, s)) return rc;
   PassArgs<T> a = base_args<T>(t, n, batch);
   int fl = 0;
   if (nmasks > 0) {
@@ -551,7 +560,7 @@ template <typename T>
 int op_spectral(int n, int lg, int batch, void* spec, const lp2d_mask* masks, int nmasks,
                 const lp2d_optics* o, void* field_out, hipStream_t s) {
   DeviceTables* t;
-  if (int rc = current_tables(&t)) return rc;
+  if (int rc = current_tables(&t, s)) return rc;
   PassArgs<T> a = base_args<T>(t, n, batch);
   int fl = F_SPEC_OUT;
   if (o) {
@@ -622,7 +631,7 @@ template <typename T>
 int op_propagate_chunk(int n, int lg, int batch, const void* in, void* out, const float* phase,
                        int64_t phase_sb, const lp2d_optics* o, void* spec, hipStream_t s) {
   DeviceTables* t;
-  if (int rc = current_tables(&t)) return rc;
+  if (int rc = current_tables(&t, s)) return rc;
   PassArgs<T> a = base_args<T>(t, n, batch);
   set_optics(a, n, o);
   int pro = 0, mid = 0;
@@ -696,7 +705,7 @@ int op_adjoint(int n, int lg, int batch, const void* gout, const void* u, const 
                int64_t phase_sb, const lp2d_optics* o, void* gin, float* gphase, int aflags,
                void* work, hipStream_t s) {
   DeviceTables* t;
-  if (int rc = current_tables(&t)) return rc;
+  if (int rc = current_tables(&t, s)) return rc;
   PassArgs<T> a = base_args<T>(t, n, batch);
   set_optics(a, n, o);
   a.z = -o->z;  // conj(H(z)) = H(-z) under the reference's trunc rule; evanescent stays 1
@@ -767,7 +776,7 @@ struct Bucket {
 template <typename T>
 int run_bucket(const Bucket& bk, hipStream_t s) {
   DeviceTables* t;
-  if (int rc = current_tables(&t)) return rc;
+  if (int rc = current_tables(&t, s)) return rc;
   PassArgs<T> a = base_args<T>(t, bk.n, bk.count);
   a.fstride = 0;
   a.fopt = bk.d_opt;
@@ -817,7 +826,7 @@ int lp2d_kz_grid(int n, const lp2d_optics* o, double* kz, void* stream) {
   if (!o || !kz) return fail(LP2D_E_ARG, "null optics or output");
   if (int rc = check_optics(o)) return rc;
   DeviceTables* t;
-  if (int rc = current_tables(&t)) return rc;
+  if (int rc = current_tables(&t, (hipStream_t)stream)) return rc;
   const double dL = o->area_size / n, k0 = 1.0 / o->wl;
   k_kz<<<grid_for((long long)n * n), 256, 0, (hipStream_t)stream>>>(kz, n, k0 * k0,
                                                                       1.0 / ((double)n * dL));
@@ -966,7 +975,7 @@ int lp2d_batch_create(const lp2d_field_desc* descs, int count, int dtype, lp2d_b
     by_lg[lg].push_back(i);
   }
   DeviceTables* t;
-  if (int rc = current_tables(&t)) return rc;
+  if (int rc = current_tables(&t, nullptr)) return rc;
   lp2d_batch* b = new lp2d_batch();
   hipGetDevice(&b->device);
   b->dtype = dtype;

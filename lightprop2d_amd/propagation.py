@@ -30,7 +30,9 @@ _tuned = False
 
 
 def _lib():
-    """Load liblp2d and, on first use, install the tuned kernel variants."""
+    """Load liblp2d and, on first use, install the tuned kernel variants and
+    initialise the library on every visible device (outside any graph capture: the
+    initialisation allocates, so a library first used under capture would fail)."""
     global _tuned
     lib = L.lib()
     if not _tuned:
@@ -40,6 +42,9 @@ def _lib():
             autotune.apply_defaults(autotune.arch())
         except L.Lp2dError:
             pass
+        if torch.cuda.is_available() and not torch.cuda.is_current_stream_capturing():
+            for d in range(torch.cuda.device_count()):
+                lib.lp2d_init(d)  # a failure resurfaces with its message at the first call
     return lib
 
 
@@ -70,6 +75,19 @@ def _field_geom(t: torch.Tensor, name="field"):
     n = t.shape[-1]
     batch = t.shape[0] if t.dim() == 3 else 1
     return n, batch, _DT[t.dtype]
+
+
+def _check_like(t: torch.Tensor, ref: torch.Tensor, name: str) -> None:
+    """A caller-supplied buffer must be exactly ``ref``'s shape, dtype and device, and
+    contiguous: the kernels read or write batch * n * n elements of ref's dtype
+    through it."""
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if t.shape != ref.shape or t.dtype != ref.dtype or t.device != ref.device:
+        raise Lp2dError(f"{name} must be a {ref.dtype} tensor of shape {tuple(ref.shape)} "
+                        f"on {ref.device}; got {t.dtype} {tuple(t.shape)} on {t.device}")
+    if not t.is_contiguous():
+        raise Lp2dError(f"{name} must be contiguous")
 
 
 def optics(area_size: float, wl: float, z: float = 0.0, aperture_radius: float = 0.0,
@@ -131,7 +149,7 @@ def fft2(field: torch.Tensor, out: Optional[torch.Tensor] = None, inverse: bool 
     if out is None:
         out = torch.empty_like(field)
     else:
-        _field_geom(out, "out")
+        _check_like(out, field, "out")
     with torch.cuda.device(field.device):
         L.check(lib.lp2d_fft2(n, batch, dt, _ptr(field), _ptr(out), int(bool(inverse)),
                               _stream(field)), "fft2")
@@ -154,7 +172,9 @@ def spatial_apply(field: torch.Tensor, masks: Sequence = (), lens_f: float = 0.0
     o = optics(area_size, wl, lens_f=lens_f) if lens_f else None
     ph, psb = _phase(phase, n, batch, field.device)
     if spectrum_out is not None:
-        _field_geom(spectrum_out, "spectrum_out")
+        _check_like(spectrum_out, field, "spectrum_out")
+        if spectrum_out.data_ptr() == field.data_ptr():
+            raise Lp2dError("spectrum_out must not alias field")
     with torch.cuda.device(field.device):
         L.check(lib.lp2d_spatial_apply(n, batch, dt, _ptr(field), marr, len(masks),
                                        C.byref(o) if o is not None else None, _ptr(ph), psb,
@@ -175,7 +195,9 @@ def spectral_apply(spectrum: torch.Tensor, masks: Sequence = (), z: float = 0.0,
     o = optics(area_size, wl, z=z, aperture_radius=aperture_radius,
                evanescent=evanescent) if (z or aperture_radius) else None
     if field_out is not None:
-        _field_geom(field_out, "field_out")
+        _check_like(field_out, spectrum, "field_out")
+        if field_out.data_ptr() == spectrum.data_ptr():
+            raise Lp2dError("field_out must not alias spectrum")
     with torch.cuda.device(spectrum.device):
         L.check(lib.lp2d_spectral_apply(n, batch, dt, _ptr(spectrum), marr, len(masks),
                                         C.byref(o) if o is not None else None,
@@ -215,7 +237,11 @@ def propagate(field: torch.Tensor, z: float, area_size: float, wl: float,
     if out is None:
         out = torch.empty_like(field)
     else:
-        _field_geom(out, "out")
+        _check_like(out, field, "out")
+    if spectrum_out is not None:
+        _check_like(spectrum_out, field, "spectrum_out")
+        if spectrum_out.data_ptr() in (field.data_ptr(), out.data_ptr()):
+            raise Lp2dError("spectrum_out must not alias the fields")
     ph, psb = _phase(phase, n, batch, field.device)
     o = optics(area_size, wl, z=z, aperture_radius=aperture_radius, lens_f=lens_f,
                evanescent=evanescent)
@@ -246,18 +272,25 @@ def propagate_adjoint(grad_out: torch.Tensor, field_in: Optional[torch.Tensor], 
     n, batch, dt = _field_geom(grad_out, "grad_out")
     if out is None:
         out = torch.empty_like(grad_out)
+    else:
+        _check_like(out, grad_out, "out")
+    if field_in is not None:
+        _check_like(field_in, grad_out, "field_in")
+        if field_in.data_ptr() == out.data_ptr():
+            raise Lp2dError("out must not alias field_in")
     if work is not None:
-        if work.shape != grad_out.shape or work.dtype != grad_out.dtype or \
-                work.device != grad_out.device or not work.is_contiguous():
-            raise Lp2dError("work must be a contiguous tensor like grad_out")
+        _check_like(work, grad_out, "work")
+        if work.data_ptr() in (out.data_ptr(), grad_out.data_ptr(),
+                               field_in.data_ptr() if field_in is not None else -1):
+            raise Lp2dError("work must not alias grad_out, field_in or out")
     elif use_workspace and grad_out.dtype == torch.complex64 and not _small_path(n, "c64"):
         work = torch.empty_like(grad_out)
     ph, psb = _phase(phase, n, batch, grad_out.device)
+    if (ph is not None or lens_f) and field_in is None:
+        raise Lp2dError("field_in is required with a phase screen or lens")
     gph = None
     if ph is not None and want_phase_grad:
         gph = torch.empty((batch, n, n), dtype=torch.float32, device=grad_out.device)
-    if field_in is not None:
-        _field_geom(field_in, "field_in")
     o = optics(area_size, wl, z=z, aperture_radius=aperture_radius, lens_f=lens_f,
                evanescent=evanescent)
     flags = 0 if want_field_grad else L.LP2D_ADJ_NO_GRAD_IN
@@ -499,6 +532,8 @@ class UnevenBatch:
                 raise Lp2dError(f"dsts[{i}] must match srcs[{i}]")
             descs[i] = L.lp2d_field_desc(n, 0, s.data_ptr(), d.data_ptr(), float(areas[i]),
                                          float(wls[i]), float(zs[i]), float(aps[i]))
+        check_disjoint_fields([(s.data_ptr(), s.numel() * s.element_size()) for s in srcs],
+                              [(d.data_ptr(), d.numel() * d.element_size()) for d in dsts])
         self._keep = (srcs, dsts)
         self.device = srcs[0].device
         self._h = C.c_void_p()
@@ -506,7 +541,6 @@ class UnevenBatch:
             L.check(lib.lp2d_batch_create(descs, m, dtype0, C.byref(self._h)), "batch_create")
         self.count = m
         self.elements = sum(s.numel() for s in srcs)
-        self._graph = None
 
     def run(self):
         with torch.cuda.device(self.device):
@@ -514,7 +548,9 @@ class UnevenBatch:
                 torch.cuda.current_stream(self.device).cuda_stream)), "batch_run")
 
     def capture_graph(self):
-        """Capture run() into a torch.cuda.CUDAGraph (HIP graph) and return it."""
+        """Capture run() into a torch.cuda.CUDAGraph (HIP graph) and return it.  The
+        graph's kernels read this batch's device tables and fields, so the graph holds a
+        reference to the batch (``graph.lp2d_batch``): replaying it never outlives them."""
         g = torch.cuda.CUDAGraph()
         s = torch.cuda.Stream(self.device)
         s.wait_stream(torch.cuda.current_stream(self.device))
@@ -522,7 +558,7 @@ class UnevenBatch:
             with torch.cuda.graph(g, stream=s):
                 self.run()
         torch.cuda.current_stream(self.device).wait_stream(s)
-        self._graph = g
+        g.lp2d_batch = self
         return g
 
     def __del__(self):
@@ -530,6 +566,44 @@ class UnevenBatch:
         if h is not None and h.value and L._lib is not None:
             L._lib.lp2d_batch_destroy(h)
             self._h = None
+
+
+def check_disjoint_fields(srcs, dsts) -> None:
+    """Fields of one uneven batch run concurrently in one launch per sweep, so a
+    destination may only share memory with its own source (in place, the same
+    address).  srcs / dsts: (address, bytes) per field.  Raises Lp2dError for a
+    destination that overlaps another field's source or destination, or its own
+    source at a different address."""
+    import bisect
+    m = len(dsts)
+    for i in range(m):
+        (s0, sb), (d0, db) = srcs[i], dsts[i]
+        if d0 != s0 and d0 < s0 + sb and s0 < d0 + db:
+            raise Lp2dError(f"dsts[{i}] overlaps srcs[{i}] without being it (in place "
+                            "needs the same tensor)")
+    order = sorted(range(m), key=lambda i: dsts[i][0])
+    for a, b in zip(order, order[1:]):
+        if dsts[b][0] < dsts[a][0] + dsts[a][1]:
+            raise Lp2dError(f"dsts[{a}] and dsts[{b}] overlap")
+    # sources may be shared (read-only); merge them into disjoint runs with their owners
+    runs = []  # [start, end, owners]
+    for i in sorted(range(m), key=lambda i: srcs[i][0]):
+        s0, sb = srcs[i]
+        if runs and s0 < runs[-1][1]:
+            runs[-1][1] = max(runs[-1][1], s0 + sb)
+            runs[-1][2].add(i)
+        else:
+            runs.append([s0, s0 + sb, {i}])
+    starts = [r[0] for r in runs]
+    for i in range(m):
+        d0, db = dsts[i]
+        k = bisect.bisect_left(starts, d0 + db)  # runs starting before the end of dst i
+        j = k - 1
+        while j >= 0 and runs[j][1] > d0:
+            if runs[j][2] - {i}:
+                other = min(runs[j][2] - {i})
+                raise Lp2dError(f"dsts[{i}] overlaps srcs[{other}] of another field")
+            j -= 1
 
 
 def _per_field(v, m):

@@ -4,6 +4,7 @@ import ctypes as C
 import os
 import re
 
+import numpy as np
 import pytest
 
 from conftest import ROOT
@@ -191,3 +192,43 @@ def test_interpolation_grids_are_the_reference_coordinates():
         assert np.array_equal(knots(d), X)
         assert np.array_equal(queries(d, n), np.sort(X * (-l1 / l2), kind="mergesort"))
         assert d.scale == -l1 / l2 and d.grid_i0 == -n // 2 and d.grid_dl == area / n
+
+
+def test_caller_buffers_must_match_the_input():
+    """Output / workspace buffers of another size, dtype or layout than the input are
+    rejected before any launch (the kernels access batch * n * n elements of the
+    input's dtype through them)."""
+    import torch
+    from lightprop2d_amd import propagation as P
+    from lightprop2d_amd._lib import Lp2dError
+    ref = torch.zeros((2, 8, 8), dtype=torch.complex64)
+    P._check_like(torch.zeros((2, 8, 8), dtype=torch.complex64), ref, "out")
+    for bad in (torch.zeros((1, 8, 8), dtype=torch.complex64),
+                torch.zeros((2, 8, 8), dtype=torch.complex128),
+                torch.zeros((2, 4, 4), dtype=torch.complex64),
+                torch.zeros((2, 8, 16), dtype=torch.complex64)[..., ::2]):
+        with pytest.raises(Lp2dError):
+            P._check_like(bad, ref, "out")
+    with pytest.raises(TypeError):
+        P._check_like(np.zeros((2, 8, 8)), ref, "out")
+
+
+def test_uneven_batch_rejects_overlapping_destinations():
+    """Destinations may share memory only with their own source, at the same address."""
+    from lightprop2d_amd.propagation import check_disjoint_fields
+    from lightprop2d_amd._lib import Lp2dError
+    MB = 1 << 20
+    srcs = [(0, MB), (MB, MB), (2 * MB, MB)]
+    check_disjoint_fields(srcs, srcs)                                   # in place
+    check_disjoint_fields(srcs, [(10 * MB, MB), (11 * MB, MB), (12 * MB, MB)])
+    check_disjoint_fields([(0, MB)] * 3, [(10 * MB, MB), (11 * MB, MB), (12 * MB, MB)])
+    with pytest.raises(Lp2dError):      # two fields write the same plane
+        check_disjoint_fields(srcs, [(10 * MB, MB), (10 * MB, MB), (12 * MB, MB)])
+    with pytest.raises(Lp2dError):      # a field writes another's source
+        check_disjoint_fields(srcs, [(0, MB), (0, MB), (2 * MB, MB)])
+    with pytest.raises(Lp2dError):      # a shared source overwritten by one of its readers
+        check_disjoint_fields([(0, MB)] * 3, [(0, MB), (11 * MB, MB), (12 * MB, MB)])
+    with pytest.raises(Lp2dError):      # a partial overlap with its own source
+        check_disjoint_fields([(0, MB)], [(MB // 2, MB)])
+    with pytest.raises(Lp2dError):      # destinations overlapping in part
+        check_disjoint_fields(srcs, [(10 * MB, MB), (10 * MB + 64, MB), (12 * MB, MB)])

@@ -168,6 +168,41 @@ def test_adjoint_dot_product_and_fd():
         assert fd == pytest.approx(gphi[iy, ix], rel=1e-5, abs=1e-8)
 
 
+def test_stack_gradient_central_differences():
+    """oracle.propagate_stack_grad against central finite differences of a real loss
+    through oracle.propagate_stack: every screen of a 4-plane stack (pixels on the
+    pupil edge and in the corners included), and the input-field gradient in both its
+    real and imaginary directions (PyTorch convention grad = dL/dRe + i dL/dIm).  A
+    dropped conjugate, a sign or a screen order swapped fails it."""
+    import oracle
+    n, area, wl, dz = 16, 120 * um, 800 * nm, 0.15 * mm
+    rs = np.random.RandomState(12)
+    u0 = rs.standard_normal((n, n)) + 1j * rs.standard_normal((n, n))
+    screens = rs.uniform(0, 2 * np.pi, (4, n, n))
+    w = rs.uniform(size=(n, n))
+
+    def loss(u, scr):
+        v = oracle.propagate_stack(u, scr, area, wl, dz)
+        return float(np.sum(w * np.abs(v) ** 2))
+
+    v = oracle.propagate_stack(u0, screens, area, wl, dz)
+    gu, gs = oracle.propagate_stack_grad(u0, screens, area, wl, dz, 2 * w * v)
+    h = 1e-6
+    for k in range(4):
+        for (iy, ix) in [(0, 0), (5, 11), (15, 15), (8, 3)]:
+            e = np.zeros_like(screens)
+            e[k, iy, ix] = h
+            fd = (loss(u0, screens + e) - loss(u0, screens - e)) / (2 * h)
+            assert fd == pytest.approx(gs[k][iy, ix], rel=1e-5, abs=1e-8), (k, iy, ix)
+    for (iy, ix) in [(2, 9), (14, 0)]:
+        e = np.zeros((n, n), dtype=complex)
+        e[iy, ix] = h
+        fd_re = (loss(u0 + e, screens) - loss(u0 - e, screens)) / (2 * h)
+        fd_im = (loss(u0 + 1j * e, screens) - loss(u0 - 1j * e, screens)) / (2 * h)
+        assert fd_re == pytest.approx(gu[iy, ix].real, rel=1e-5, abs=1e-8)
+        assert fd_im == pytest.approx(gu[iy, ix].imag, rel=1e-5, abs=1e-8)
+
+
 # ----------------------------------------------------------------- analytic physics
 def test_gaussian_beam_analytic():
     """Reference tests/gaussbeam_diffraction.py:51-79 formulas: on-axis intensity
