@@ -260,6 +260,26 @@ int lp2d_propagate_adjoint_ws(int n, int batch, int dtype, const void* grad_out,
                               const lp2d_optics* optics, void* grad_in, float* grad_phase,
                               int adj_flags, void* work, void* stream);
 
+/* lp2d_propagate_adjoint for callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.  No
+ * counterpart in the reference (it has no gradients); same arguments and results as
+ * lp2d_propagate_adjoint (no workspace: the intermediate plane stays in grad_in),
+ * except that with a shared screen (phase_stride_b = 0) over a batch, grad_phase is a
+ * single n x n plane receiving sum_b gphi[b]; the per-field or per-group planes of
+ * that sum live in a 16 MiB device array of the library (one per device), so the call
+ * allocates nothing.  Limits: those planes must fit in 16 MiB (batch x n^2 floats for
+ * sizes that run in one whole-field launch, e.g. batch <= 256 at 128^2 complex64;
+ * otherwise about 8 elements x 512K threads: every size fits), else LP2D_E_SIZE with
+ * the figure.  lp2d_init(device) must have run (LP2D_E_INIT otherwise; it is what
+ * allocates).  Calls on one device share that array: two of them that overlap in time
+ * (two streams, two threads) race and must be ordered by the caller.  The array keeps
+ * the last call's planes; every plane a call reads it has written first. */
+int lp2d_propagate_adjoint_noalloc(int n, int batch, int dtype, const void* grad_out,
+                                   const void* field_in, const float* phase,
+                                   int64_t phase_stride_b, const lp2d_optics* optics,
+                                   void* grad_in, float* grad_phase, int adj_flags,
+                                   void* stream);
+
 /* Sum `batch` float planes of `count` elements: out[i] = sum_b in[b*count + i].
  * Used to reduce per-field phase gradients of a shared mask. */
 int lp2d_sum_planes(int64_t count, int batch, const float* in, float* out, void* stream);

@@ -15,6 +15,13 @@
 
 using namespace lp2d;
 
+// Scratch of the allocation-free adjoint (lp2d_propagate_adjoint_noalloc): the
+// per-group partial sums of a shared screen's phase gradient.  One copy per device,
+// held until the process ends; host code reaches it only through the address that
+// lp2d_init takes from hipGetSymbolAddress on each device.
+constexpr size_t kScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) float scratch[kScratchBytes / sizeof(float)];
+
 namespace lp2d {
 int resident_groups(const void* kernel, int threads) {
   int dev = 0, cus = 0, nb = 0;
@@ -54,6 +61,7 @@ constexpr int kMaxDevices = 64;
 struct DeviceTables {
   cx<float>* twf = nullptr;
   cx<double>* twd = nullptr;
+  float* scratch = nullptr;  // this device's copy of `scratch`
   bool ready = false;
 };
 std::mutex g_mu;
@@ -115,10 +123,15 @@ int init_device(int device) {
     rc = hip_fail(e, "twiddle table allocation");
   } else {
     k_init_twiddles<<<(kTwiddleLen + 255) / 256, 256, 0, s>>>(d.twf, d.twd);
+    void* sp = nullptr;
     if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(s)) != hipSuccess)
       rc = hip_fail(e, "twiddle table init");
-    else
+    else if ((e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch))) != hipSuccess || !sp)
+      rc = hip_fail(e, "scratch address");
+    else {
+      d.scratch = (float*)sp;
       d.ready = true;
+    }
   }
   if (s) hipStreamDestroy(s);
   if (prev != device) hipSetDevice(prev);
@@ -667,19 +680,31 @@ int op_propagate_chunk(int n, int lg, int batch, const void* in, void* out, cons
   return run_rows<T>(a, lg, true, s);
 }
 
+// Field groups of the gradient pass over a batch: about 512K threads in flight,
+// whatever the tile height; the kernel gives group g the fields [g*per,
+// min(batch, (g+1)*per)), so the count is recomputed from per and no group is empty
+// (batch 5 in 4 groups of 2 would leave group 3 at field 6).
 template <typename T>
-int run_rows_grad(const PassArgs<T>& a, int lg, float* gphase, hipStream_t s) {
+int grad_groups(int lg, int batch) {
   const RowVar<T>& v = grad_variant<T>(lg);
   const int n = 1 << lg;
   const int tpf = (n + v.grad_lines - 1) / v.grad_lines;
-  // about 512K threads in flight, whatever the tile height
   int groups = (2048 * 256 / ((n / v.E) * v.grad_lines)) / tpf;
   if (groups < 1) groups = 1;
-  if (groups > a.batch) groups = a.batch;
-  // the kernel gives group g the fields [g*per, min(batch, (g+1)*per)): recount the
-  // groups so that none is empty (batch 5 in 4 groups of 2 would leave group 3 at 6)
-  const int per = (a.batch + groups - 1) / groups;
-  groups = (a.batch + per - 1) / per;
+  if (groups > batch) groups = batch;
+  const int per = (batch + groups - 1) / groups;
+  return (batch + per - 1) / per;
+}
+
+// The gradient pass with a shared screen: each group of fields sums its phase
+// gradients into one n x n plane of `partial` (planes 0 .. groups-1), and a second
+// kernel adds the planes into `out` (which may be plane 0 of partial).  With a single
+// group the pass writes `out` directly.
+template <typename T>
+int run_rows_grad(const PassArgs<T>& a, int lg, float* partial, float* out, hipStream_t s) {
+  const RowVar<T>& v = grad_variant<T>(lg);
+  const int n = 1 << lg;
+  const int groups = grad_groups<T>(lg, a.batch);
   std::string nm;
   if (g_trace_on) {
     char b[64];
@@ -689,21 +714,24 @@ int run_rows_grad(const PassArgs<T>& a, int lg, float* gphase, hipStream_t s) {
   hipError_t e;
   {
     TraceScope ts(s, nm);
-    e = v.grad_fn(a, groups, gphase, s);
+    e = v.grad_fn(a, groups, groups > 1 ? partial : out, s);
   }
   if (e != hipSuccess) return hip_fail(e, "adjoint gradient pass launch");
   if (groups > 1) {
     const long long cnt = (long long)n * n;
-    k_sum_planes<<<grid_for(cnt), 256, 0, s>>>(gphase, gphase, cnt, groups);
+    k_sum_planes<<<grid_for(cnt), 256, 0, s>>>(partial, out, cnt, groups);
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "sum kernel launch");
   }
   return LP2D_OK;
 }
 
 template <typename T>
+// scratch (allocation-free form): with a shared screen over a batch, gphase is one
+// n x n plane and the per-field or per-group gradient planes go to scratch; otherwise
+// (scratch == nullptr) they go to gphase's planes and the sum to its plane 0.
 int op_adjoint(int n, int lg, int batch, const void* gout, const void* u, const float* phase,
                int64_t phase_sb, const lp2d_optics* o, void* gin, float* gphase, int aflags,
-               void* work, hipStream_t s) {
+               void* work, hipStream_t s, float* scratch = nullptr) {
   DeviceTables* t;
   if (int rc = current_tables(&t, s)) return rc;
   PassArgs<T> a = base_args<T>(t, n, batch);
@@ -722,12 +750,14 @@ int op_adjoint(int n, int lg, int batch, const void* gout, const void* u, const 
     epi |= F_GRAD;
   }
   const bool shared_sum = gphase && phase_sb == 0 && batch > 1;
+  float* planes = (shared_sum && scratch) ? scratch : gphase;  // per-field / per-group planes
   const T sc = (T)(1.0 / ((double)n * n));
   a.scale = sc;
   a.src = (const cx<T>*)gout;
   a.dst = (cx<T>*)gin;
   if (use_small<T>(lg)) {
     a.flags = mid | epi;
+    if (shared_sum) a.grad_phase = planes;
     if (int rc = run_small<T>(a, lg, MODE_FWD | MODE_INV, s)) return rc;
   } else {
     a.flags = mid;
@@ -747,14 +777,14 @@ int op_adjoint(int n, int lg, int batch, const void* gout, const void* u, const 
     }
     if (shared_sum) {
       a.flags = F_SCALE | no_store | tl;
-      return run_rows_grad<T>(a, lg, gphase, s);
+      return run_rows_grad<T>(a, lg, planes, gphase, s);
     }
     a.flags = epi | tl;
     if (int rc = run_rows<T>(a, lg, true, s)) return rc;
   }
-  if (shared_sum) {  // per-field planes from the whole-field kernel: reduce into plane 0
+  if (shared_sum) {  // per-field planes from the whole-field kernel: reduce into gphase
     const long long cnt = (long long)n * n;
-    k_sum_planes<<<grid_for(cnt), 256, 0, s>>>(gphase, gphase, cnt, batch);
+    k_sum_planes<<<grid_for(cnt), 256, 0, s>>>(planes, gphase, cnt, batch);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "sum kernel launch");
   }
@@ -957,6 +987,47 @@ int lp2d_propagate_adjoint_ws(int n, int batch, int dtype, const void* grad_out,
   return dtype == LP2D_C64
              ? op_adjoint<float>(n, lg, batch, grad_out, field_in, phase, phase_stride_b, optics, grad_in, grad_phase, adj_flags, work, s)
              : op_adjoint<double>(n, lg, batch, grad_out, field_in, phase, phase_stride_b, optics, grad_in, grad_phase, adj_flags, work, s);
+}
+
+int lp2d_propagate_adjoint_noalloc(int n, int batch, int dtype, const void* grad_out,
+                                   const void* field_in, const float* phase,
+                                   int64_t phase_stride_b, const lp2d_optics* optics,
+                                   void* grad_in, float* grad_phase, int adj_flags,
+                                   void* stream) {
+  int lg;
+  if (int rc = check_n(n, batch, dtype, &lg)) return rc;
+  if (!grad_out || !grad_in || !optics) return fail(LP2D_E_ARG, "null pointer");
+  if (int rc = check_optics(optics)) return rc;
+  if (grad_phase && !phase) return fail(LP2D_E_ARG, "grad_phase requires phase");
+  if ((phase || optics->lens_f != 0.0) && !field_in)
+    return fail(LP2D_E_ARG, "field_in is required with a phase screen or lens");
+  if (field_in && grad_in == field_in) return fail(LP2D_E_ARG, "grad_in must not alias field_in");
+  if (adj_flags & ~LP2D_ADJ_NO_GRAD_IN) return fail(LP2D_E_ARG, "unknown adj_flags bits");
+  ensure_registry();
+  // scratch needed: the per-field planes of the whole-field kernel, or the per-group
+  // planes of the gradient pass, of a shared screen over a batch
+  size_t need = 0;
+  if (grad_phase && phase_stride_b == 0 && batch > 1) {
+    const size_t plane = (size_t)n * n * sizeof(float);
+    const bool small = dtype == LP2D_C64 ? use_small<float>(lg) : use_small<double>(lg);
+    const int groups = dtype == LP2D_C64 ? grad_groups<float>(lg, batch) : grad_groups<double>(lg, batch);
+    need = small ? (size_t)batch * plane : (groups > 1 ? (size_t)groups * plane : 0);
+  }
+  if (need > kScratchBytes)
+    return fail(LP2D_E_SIZE,
+                "n=%d batch=%d: the shared-screen gradient needs %zu bytes of scratch, the "
+                "allocation-free adjoint holds %zu (use lp2d_propagate_adjoint)",
+                n, batch, need, kScratchBytes);
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices || !g_dev[dev].ready || !g_dev[dev].scratch)
+    return fail(LP2D_E_INIT, "call lp2d_init(%d) before lp2d_propagate_adjoint_noalloc", dev);
+  float* sp = g_dev[dev].scratch;
+  hipStream_t s = (hipStream_t)stream;
+  return dtype == LP2D_C64
+             ? op_adjoint<float>(n, lg, batch, grad_out, field_in, phase, phase_stride_b, optics, grad_in, grad_phase, adj_flags, nullptr, s, sp)
+             : op_adjoint<double>(n, lg, batch, grad_out, field_in, phase, phase_stride_b, optics, grad_in, grad_phase, adj_flags, nullptr, s, sp);
 }
 
 int lp2d_batch_create(const lp2d_field_desc* descs, int count, int dtype, lp2d_batch** out) {

@@ -16,7 +16,7 @@ from . import _lib as L
 from ._lib import Lp2dError
 
 __all__ = ["fft2", "ifft2", "kz_grid", "spatial_apply", "spectral_apply", "propagate",
-           "propagate_adjoint", "intensity", "sum_planes", "UnevenBatch", "optics",
+           "propagate_adjoint", "propagate_adjoint_noalloc", "intensity", "sum_planes", "UnevenBatch", "optics",
            "expand", "coarse", "mode_matrix", "modes_project", "modes_construct",
            "Lp2dError"]
 
@@ -302,6 +302,45 @@ def propagate_adjoint(grad_out: torch.Tensor, field_in: Optional[torch.Tensor], 
     if gph is not None and psb == 0:
         gph = gph[0]
     return (out if want_field_grad else None), gph
+
+
+def propagate_adjoint_noalloc(grad_out: torch.Tensor, field_in: Optional[torch.Tensor],
+                              z: float, area_size: float, wl: float, out: torch.Tensor,
+                              phase: Optional[torch.Tensor] = None,
+                              grad_phase: Optional[torch.Tensor] = None,
+                              aperture_radius: float = 0.0, lens_f: float = 0.0,
+                              evanescent: str = "reference", want_field_grad: bool = True):
+    """`propagate_adjoint` without any device allocation (lp2d_propagate_adjoint_noalloc),
+    for code under graph capture: the caller passes `out` (grad_out's shape) and, for a
+    phase gradient, `grad_phase` of the phase's shape ((n, n) for a screen shared by the
+    batch, summed over it in the library's own scratch array).  Calls on one device
+    share that array, so they must not overlap in time."""
+    lib = _lib()
+    n, batch, dt = _field_geom(grad_out, "grad_out")
+    _check_like(out, grad_out, "out")
+    if field_in is not None:
+        _check_like(field_in, grad_out, "field_in")
+        if field_in.data_ptr() == out.data_ptr():
+            raise Lp2dError("out must not alias field_in")
+    ph, psb = _phase(phase, n, batch, grad_out.device)
+    if (ph is not None or lens_f) and field_in is None:
+        raise Lp2dError("field_in is required with a phase screen or lens")
+    if grad_phase is not None:
+        if ph is None:
+            raise Lp2dError("grad_phase requires phase")
+        if grad_phase.shape != ph.shape or grad_phase.dtype != torch.float32 or \
+                grad_phase.device != grad_out.device or not grad_phase.is_contiguous():
+            raise Lp2dError(f"grad_phase must be a contiguous float32 tensor of shape "
+                            f"{tuple(ph.shape)} on {grad_out.device}")
+    o = optics(area_size, wl, z=z, aperture_radius=aperture_radius, lens_f=lens_f,
+               evanescent=evanescent)
+    flags = 0 if want_field_grad else L.LP2D_ADJ_NO_GRAD_IN
+    with torch.cuda.device(grad_out.device):
+        L.check(lib.lp2d_propagate_adjoint_noalloc(n, batch, dt, _ptr(grad_out), _ptr(field_in),
+                                                   _ptr(ph), psb, C.byref(o), _ptr(out),
+                                                   _ptr(grad_phase), flags, _stream(grad_out)),
+                "propagate_adjoint_noalloc")
+    return out, grad_phase
 
 
 def sum_planes(planes: torch.Tensor, out: torch.Tensor):

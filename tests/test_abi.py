@@ -232,3 +232,20 @@ def test_uneven_batch_rejects_overlapping_destinations():
         check_disjoint_fields([(0, MB)], [(MB // 2, MB)])
     with pytest.raises(Lp2dError):      # destinations overlapping in part
         check_disjoint_fields(srcs, [(10 * MB, MB), (10 * MB + 64, MB), (12 * MB, MB)])
+
+
+def test_noalloc_adjoint_rejects_sizes_beyond_its_scratch(lib):
+    """The allocation-free adjoint checks the scratch its call needs before any device
+    work: a shared screen over 300 fields of 128^2 (whole-field kernel, one gradient
+    plane per field: 300 x 64 KiB) exceeds the 16 MiB array."""
+    from lightprop2d_amd import _lib
+    p = C.c_void_p(16)
+    o = _lib.lp2d_optics(1.0, 1e-4, 0.1, 0.0, 0.0, 0, 0)
+    rc = lib.lp2d_propagate_adjoint_noalloc(128, 300, 0, p, C.c_void_p(32), C.c_void_p(48), 0,
+                                            C.byref(o), C.c_void_p(64), C.c_void_p(80), 0, None)
+    assert rc == _lib.LP2D_E_SIZE
+    assert b"scratch" in lib.lp2d_last_error()
+    assert lib.lp2d_propagate_adjoint_noalloc(64, 1, 0, p, None, p, 0, C.byref(o), p, None, 0,
+                                              None) == _lib.LP2D_E_ARG
+    assert lib.lp2d_propagate_adjoint_noalloc(64, 1, 0, p, p, None, 0, C.byref(o),
+                                              C.c_void_p(32), p, 0, None) == _lib.LP2D_E_ARG
