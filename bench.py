@@ -410,6 +410,14 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     if args.leg:
+        # a plain 256 MiB device copy first: the calibration dispatch for rocprof's
+        # FETCH_SIZE / WRITE_SIZE counters in a profiled run of one leg
+        a = torch.empty(1 << 25, dtype=torch.complex64, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
         print(json.dumps({args.leg: LEGS[args.leg](torch, P, L, lib, dev)}), flush=True)
         return
     n, B = args.n, args.batch

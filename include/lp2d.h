@@ -232,6 +232,21 @@ int lp2d_propagate(int n, int batch, int dtype, const void* field_in, void* fiel
                    const float* phase, int64_t phase_stride_b, const lp2d_optics* optics,
                    void* spectrum_out, void* stream);
 
+/* Multi-plane stack: u_0 = field_in, u_{k+1} = the lp2d_propagate of u_k with phase
+ * screens[k] (optics->lens_f applied on every plane), k = 0 .. nscreens-1, and
+ * field_out = u_nscreens.  screens: nscreens float32 (n, n) planes (radians, shared
+ * by the batch), plane k at screens + k*n*n.  planes_out: NULL, or nscreens host-side
+ * pointers of which planes_out[k] (1 <= k < nscreens), when not NULL, receives u_k
+ * (batch, n, n; must not alias the fields); planes_out[0] is ignored.  field_in ==
+ * field_out allowed.  For complex64 sizes with a block-tiled intermediate the inverse
+ * row pass of plane k, the screen of plane k+1 and its forward row pass run as one
+ * sweep (2*nscreens + 1 sweeps in all).  Reference sequence per plane:
+ * coordinate_filter(f_init=exp(i*phi_k)) then propagate(z) (beam2d.py:339-342,
+ * 481-490). */
+int lp2d_propagate_stack(int n, int batch, int dtype, const void* field_in, void* field_out,
+                         const float* screens, int nscreens, const lp2d_optics* optics,
+                         void* const* planes_out, void* stream);
+
 /* Adjoint (vector-Jacobian product) of lp2d_propagate, for a real loss with
  * PyTorch's complex-gradient convention (grad = dL/dRe + i dL/dIm):
  *   g1 = ifft2(conj(H) * aperture * fft2(grad_out))

@@ -16,7 +16,9 @@ has no gradients; the forward is the sequence coordinate_filter(exp(i*phase)) ->
 
 ``propagate_stack`` chains K phase screens with checkpointing of the intermediate
 planes: only every s-th plane input is kept in the forward; the backward recomputes
-each segment from its checkpoint before running the adjoints in reverse.
+each segment from its checkpoint before running the adjoints in reverse.  Forward and
+recomputation run as fused stacks (lp2d_propagate_stack: 2K + 1 sweeps for K planes,
+the checkpoints stored by the chained row passes).
 """
 from __future__ import annotations
 
@@ -96,12 +98,15 @@ class StackFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, u0, screens, dz, area_size, wl, stride, recompute):
         K = screens.shape[0]
-        u = u0.contiguous()
-        ckpts = {}
-        for k in range(K):
-            if k % stride == 0:
-                ckpts[k] = u
-            u = P.propagate(u, dz, area_size, wl, phase=screens[k])
+        u0 = u0.contiguous()
+        screens_c = screens.contiguous()
+        ckpts = {0: u0}
+        planes = [None] * K
+        for k in range(stride, K, stride):
+            planes[k] = torch.empty_like(u0)
+            ckpts[k] = planes[k]
+        # one fused stack call: the chained row passes store the checkpointed inputs
+        u = P.propagate_stack(u0, screens_c, dz, area_size, wl, planes=planes)
         ctx.ckpts = ckpts
         ctx.cfg = (dz, area_size, wl, stride, K, recompute)
         ctx.save_for_backward(screens)
@@ -115,19 +120,25 @@ class StackFn(torch.autograd.Function):
         g = grad_out.contiguous()
         gscreens = torch.zeros_like(screens) if ctx.needs_input_grad[1] else None
 
+        screens_c = screens.contiguous()
+
         def advance(u, k0, k1):  # plane input k1 from plane input k0
-            for k in range(k0, k1):
-                u = P.propagate(u, dz, area, wl, phase=screens[k])
-            return u
+            if k1 <= k0:
+                return u
+            return P.propagate_stack(u, screens_c[k0:k1], dz, area, wl)
 
         for seg0 in reversed(range(0, K, stride)):
             seg1 = min(K, seg0 + stride)
             ck = ctx.ckpts[seg0]
             planes = None
             if recompute == "segment":
+                m = seg1 - seg0
                 planes = [ck]
-                for k in range(seg0, seg1 - 1):
-                    planes.append(advance(planes[-1], k, k + 1))
+                if m > 1:  # the segment's plane inputs from one stack call
+                    pl = [None] + [torch.empty_like(ck) for _ in range(m - 2)]
+                    last = P.propagate_stack(ck, screens_c[seg0:seg1 - 1], dz, area, wl,
+                                             planes=pl)
+                    planes += pl[1:] + [last]
             for k in reversed(range(seg0, seg1)):
                 uk = planes[k - seg0] if planes is not None else advance(ck, seg0, k)
                 need_g = k > 0 or ctx.needs_input_grad[0]

@@ -73,6 +73,16 @@ __global__ __launch_bounds__((N / E) * LINES) LP2D_OCC void k_rows_grad(PassArgs
   }
 }
 
+// Chained-plane row pass: one workgroup per row tile of LINES rows of one field.
+template <typename T, int N, int E, int LINES, int PS>
+__global__ __launch_bounds__((N / E) * LINES) LP2D_OCC void k_rows_chain(PassArgs<T> a) {
+  __shared__ cx<T> lds[LINES * RowPitch<N, PS, LINES>::value];
+  constexpr int TPF = N / LINES;
+  const int tile = blockIdx.x;
+  const int b = tile / TPF;
+  rows_chain_body<T, N, E, LINES, PS>(a, b, (tile - b * TPF) * LINES, lds, threadIdx.x);
+}
+
 // Grid: (N/W strips) x (field groups of `per` fields).
 template <typename T, int N, int E, int W, int MODE, int PS, int MIDK>
 __global__ __launch_bounds__((N / E) * W) LP2D_OCC void k_cols(PassArgs<T> a, int per) {
@@ -261,6 +271,29 @@ hipError_t launch_rows_grad(const PassArgs<T>& a, int groups, float* partial, hi
   return hipGetLastError();
 }
 
+// The chained-plane pass runs on block-tiled planes of whole tiles only (complex64
+// schedules whose row tile is a whole number of row blocks and whose threads cover
+// whole tile widths); other shapes report hipErrorInvalidValue and the caller runs
+// the plain sweeps.
+template <typename T, int N, int E, int LINES, int PS>
+constexpr bool chain_ok() {
+  return sizeof(T) == 4 && N % LINES == 0 && TiledRowMap<N, N / E, LINES>::ok;
+}
+template <typename T, int N, int E, int LINES, int PS>
+hipError_t launch_rows_chain(const PassArgs<T>& a, hipStream_t s) {
+  if constexpr (chain_ok<T, N, E, LINES, PS>()) {
+    if (!(a.flags & F_TILED) || !a.phase || a.src_tab || a.dst_tab) return hipErrorInvalidValue;
+    constexpr int threads = (N / E) * LINES;
+    const unsigned grid = (unsigned)((long long)a.batch * (N / LINES));
+    k_rows_chain<T, N, E, LINES, PS><<<grid, threads, 0, s>>>(a);
+    return hipGetLastError();
+  } else {
+    (void)a;
+    (void)s;
+    return hipErrorInvalidValue;
+  }
+}
+
 // Fields per column workgroup: share the transfer function across a group of fields
 // while keeping at least ~kColBlocks workgroups in flight.
 constexpr int kColBlocks = 1024;
@@ -418,7 +451,10 @@ void add_row(Registry<T>& r) {
   // threads, measured 1156 us against 749 us for one row per workgroup)
   r.rows[lg][k++] = RowVar<T>{&launch_rows<T, N, E, S::LINES, S::PS>,
                               &launch_rows_grad<T, N, E, S::LINES, S::PS, (E <= 8)>, E,
-                              S::LINES, S::THREADS, S::LDS, S::LINES, 0};
+                              S::LINES, S::THREADS, S::LDS, S::LINES, 0,
+                              chain_ok<T, N, E, S::LINES, S::PS>()
+                                  ? &launch_rows_chain<T, N, E, S::LINES, S::PS>
+                                  : nullptr};
 }
 
 template <typename T, int N, int E>
@@ -431,7 +467,10 @@ void add_row_pipe(Registry<T>& r) {
   if constexpr (lds <= kLdsBudget)
     r.rows[lg][k++] = RowVar<T>{&launch_rows_pipe<T, N, E, S::LINES, S::PS>,
                                 &launch_rows_grad<T, N, E, S::LINES, S::PS, (E <= 8)>, E,
-                                S::LINES, S::THREADS, lds, S::LINES, 2};
+                                S::LINES, S::THREADS, lds, S::LINES, 2,
+                                chain_ok<T, N, E, S::LINES, S::PS>()
+                                    ? &launch_rows_chain<T, N, E, S::LINES, S::PS>
+                                    : nullptr};
 }
 
 template <typename T, int N, int E, int WT, bool HOIST = false>

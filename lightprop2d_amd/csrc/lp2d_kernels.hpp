@@ -77,6 +77,31 @@ struct TiledWaveMap {
     t = w * (64 / TR) + (lam / (TR * TC)) * TC + lam % TC;
   }
 };
+// Thread -> (row l, lane t) for a row tile of LINES rows that reads (or writes) the
+// block-tiled plane and writes (or reads) the natural one.  With 8 x 8 tiles and TL a
+// multiple of 16, each wave takes 4 rows x 16 lanes of an 8-row block: every access
+// of the tiled plane covers two 256-byte runs and every access of the natural plane
+// whole 128-byte lines (with 8 x 8 lanes per wave the two halves of a natural line
+// would come from two instructions, and the line would be written back twice).
+// Otherwise TiledWaveMap (whole tiles per instruction).
+template <int N, int TL, int LINES>
+struct TiledRowMap {
+  static constexpr int TR = tile_rows(N), TC = tile_cols(N);
+  using WM = TiledWaveMap<TL, TR, TC>;
+  static constexpr bool LINE16 = TR == 8 && TC == 8 && TL % 16 == 0;
+  static constexpr bool ok = LINES % TR == 0 && TL % TC == 0 && (LINE16 || WM::ok);
+  static __device__ __forceinline__ void map(int tid, int& l, int& t) {
+    if constexpr (LINE16) {
+      const int g = tid / (TR * TL), r = tid - g * (TR * TL);
+      const int wv = r >> 6, ln = r & 63;
+      l = g * TR + (ln >> 4) + 4 * (wv & 1);
+      t = (ln & 15) + 16 * (wv >> 1);
+    } else {
+      WM::map(tid, l, t);
+    }
+  }
+};
+
 constexpr int F_SPATIAL = F_MASK0 | F_MASK1 | F_LENS | F_PHASE | F_FIELD_OUT;
 constexpr int F_MID = F_H | F_APERTURE | F_SMASK0 | F_SMASK1 | F_SPEC_OUT;
 
@@ -202,6 +227,12 @@ __device__ __forceinline__ void gst(LP2D_GLB cx<T>* p, cx<T> x) {
     *p = x;
   }
 }
+
+// Nontemporal loads of a natural-order row only when the TL lanes of one row cover
+// whole 128-byte lines: the hint keeps a line out of the caches, so a line read in two
+// halves by two instructions is fetched twice (measured 1.23 x at 256 points, 8 lanes).
+template <typename T, int TL>
+constexpr int nt_row_site() { return TL * (int)sizeof(cx<T>) >= 128 ? NT_ROW_LD : 0; }
 
 // Base of field b's source / destination plane (b uniform across the workgroup).
 template <typename T>
@@ -529,8 +560,8 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
   // so every load instruction of the wave reads whole tiles (8 x 8: one 512-byte tile).
   // (The same map for the forward pass of 2 x 4 tiles measured no different.)
   int l = tid / TL, t = tid - (tid / TL) * TL;
-  if constexpr (INV && LINES % TR == 0 && TL % TC == 0 && WM::ok) {
-    if (a.flags & F_TILED) WM::map(tid, l, t);
+  if constexpr (INV && TiledRowMap<N, TL, LINES>::ok) {
+    if (a.flags & F_TILED) TiledRowMap<N, TL, LINES>::map(tid, l, t);
   }
   const int y = valid_row(y0 + l, y0 + nrows - 1);
   const bool valid = l < nrows;
@@ -549,7 +580,7 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
     for (int m = 0; m < E; ++m) v[m] = gld<NT_ROW_LD>(src + (lo_t + (unsigned)(tile_rows(N) * TL * m)));
   } else {
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = gld<NT_ROW_LD>(src + (lo + (unsigned)(TL * m)));
+    for (int m = 0; m < E; ++m) v[m] = gld<nt_row_site<T, TL>()>(src + (lo + (unsigned)(TL * m)));
   }
   if constexpr (FEAT) {
     if (a.flags & F_SPATIAL) {
@@ -588,6 +619,49 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
         if (valid) gst<INV ? NT_ROWI_ST : NT_ROW_ST>(dst + (lo + (unsigned)(TL * m)), mk(v[m].re * sc, v[m].im * sc));
     }
   }
+}
+
+// Chained planes (a multi-plane stack u_{k+1} = P(dz)[u_k exp(i phi_k)]): the inverse
+// row pass of plane k and the forward row pass of plane k+1 in one sweep over the
+// block-tiled intermediate (F_TILED): load the tile, inverse row FFT, scale (1/n^2),
+// store the plane u_{k+1} in natural order when a.field_out is set (F_FIELD_OUT, a
+// checkpoint), multiply by exp(i*phase[y, x]) (the next screen), forward row FFT,
+// store the tile in place.  The thread map (TiledRowMap) makes every tiled access
+// cover whole tiles or 256-byte runs and every natural-order store whole 128-byte
+// lines.  Three sweeps per plane become two.
+template <typename T, int N, int E, int LINES, int PS>
+__device__ __forceinline__ void rows_chain_body(const PassArgs<T>& a, int b, int y0,
+                                                cx<T>* lds, int tid) {
+  constexpr int TL = N / E;
+  constexpr int L = RowPitch<N, PS, LINES>::value;
+  constexpr int TR = tile_rows(N), TC = tile_cols(N);
+  using RM = TiledRowMap<N, TL, LINES>;
+  static_assert(RM::ok && N % LINES == 0, "whole tiles of whole row blocks");
+  int l, t;
+  RM::map(tid, l, t);
+  const int y = y0 + l;
+  const long long ubase = (long long)y0 * N;
+  const unsigned lo_t = tiled_off<N>(l, t);
+  constexpr unsigned TS = TR * TL;  // tiled offset step of element m
+  cx<T> v[E];
+  const LP2D_GLB cx<T>* __restrict__ src = field_src(a, b) + ubase;
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = gld<NT_ROW_LD>(src + (lo_t + TS * (unsigned)m));
+  LineAcc<T, PS> acc{lds + l * L};
+  line_fft<T, N, E, true>(v, t, acc, a.tw_inv + (N - 2));
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = cscale(v[m], a.scale);
+  if (a.flags & F_FIELD_OUT) {
+    LP2D_GLB cx<T>* __restrict__ ck =
+        glb(uniform_ptr(a.field_out + (long long)b * a.fstride)) + (long long)y * N;
+#pragma unroll
+    for (int m = 0; m < E; ++m) gst<NT_ROWI_ST>(ck + (t + TL * m), v[m]);
+  }
+  phase_mul_all<T, E, false>(a, v, b, y, t, TL, N);
+  line_fft<T, N, E, false>(v, t, acc, a.tw + (N - 2));
+  LP2D_GLB cx<T>* __restrict__ dst = field_dst(a, b) + ubase;
+#pragma unroll
+  for (int m = 0; m < E; ++m) gst<NT_ROW_ST>(dst + (lo_t + TS * (unsigned)m), v[m]);
 }
 
 // Row tiles of LINES rows with software pipelining: the tile of step k+1 is loaded
@@ -645,7 +719,8 @@ __device__ __forceinline__ void rows_pipe_body(const PassArgs<T>& a, int u0, int
   auto fetch = [&](int u) {
     const LP2D_GLB cx<T>* __restrict__ src = glb(uniform_ptr(a.src + tile_off(u)));
 #pragma unroll
-    for (int m = 0; m < E; ++m) nxt[m] = gld<NT_ROW_LD>(src + (ld_o + LD_S * (unsigned)m));
+    for (int m = 0; m < E; ++m)
+      nxt[m] = gld<LD_T ? NT_ROW_LD : nt_row_site<T, TL>()>(src + (ld_o + LD_S * (unsigned)m));
   };
   auto step = [&](int u) {
 #pragma unroll

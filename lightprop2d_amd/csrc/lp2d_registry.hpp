@@ -18,6 +18,8 @@ using SmallLaunch = hipError_t (*)(const PassArgs<T>&, int mode, hipStream_t);
 template <typename T>
 using RowGradLaunch = hipError_t (*)(const PassArgs<T>&, int groups, float* partial,
                                      hipStream_t);
+template <typename T>
+using RowChainLaunch = hipError_t (*)(const PassArgs<T>&, hipStream_t);
 
 template <typename T>
 struct RowVar {
@@ -26,6 +28,7 @@ struct RowVar {
   int E, lines, threads, lds;
   int grad_lines;  // row tile of grad_fn: whole 8-row blocks where the threads allow
   int kind;        // bit 1: software-pipelined over the row tiles of the batch
+  RowChainLaunch<T> chain_fn;  // inverse rows + next screen + forward rows (tiled)
 };
 template <typename T>
 struct ColVar {

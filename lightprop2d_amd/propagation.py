@@ -16,7 +16,7 @@ from . import _lib as L
 from ._lib import Lp2dError
 
 __all__ = ["fft2", "ifft2", "kz_grid", "spatial_apply", "spectral_apply", "propagate",
-           "propagate_adjoint", "propagate_adjoint_noalloc", "intensity", "sum_planes", "UnevenBatch", "optics",
+           "propagate_adjoint", "propagate_adjoint_noalloc", "propagate_stack", "intensity", "sum_planes", "UnevenBatch", "optics",
            "expand", "coarse", "mode_matrix", "modes_project", "modes_construct",
            "Lp2dError"]
 
@@ -248,6 +248,50 @@ def propagate(field: torch.Tensor, z: float, area_size: float, wl: float,
     with torch.cuda.device(field.device):
         L.check(lib.lp2d_propagate(n, batch, dt, _ptr(field), _ptr(out), _ptr(ph), psb,
                                    C.byref(o), _ptr(spectrum_out), _stream(field)), "propagate")
+    return out
+
+
+def propagate_stack(field: torch.Tensor, screens: torch.Tensor, z: float, area_size: float,
+                    wl: float, aperture_radius: float = 0.0, lens_f: float = 0.0,
+                    evanescent: str = "reference", out: Optional[torch.Tensor] = None,
+                    planes: Optional[Sequence[Optional[torch.Tensor]]] = None):
+    """Multi-plane stack u_{k+1} = propagate(u_k, z, phase=screens[k]) for the K float32
+    screens (K, n, n) shared by the batch; returns u_K.  planes[k] (1 <= k < K), when
+    given, receives u_k (checkpoints).  Complex64 sizes with a block-tiled intermediate
+    run 2K + 1 sweeps: the inverse row pass of a plane, the next screen and the next
+    forward row pass are one sweep."""
+    lib = _lib()
+    n, batch, dt = _field_geom(field)
+    if not isinstance(screens, torch.Tensor) or screens.dtype != torch.float32 or \
+            screens.dim() != 3 or screens.shape[1:] != (n, n) or \
+            screens.device != field.device or not screens.is_contiguous():
+        raise Lp2dError(f"screens must be a contiguous float32 (K, {n}, {n}) tensor on the "
+                        "field's device")
+    K = screens.shape[0]
+    if K < 1:
+        raise Lp2dError("at least one screen")
+    if out is None:
+        out = torch.empty_like(field)
+    else:
+        _check_like(out, field, "out")
+    parr = None
+    if planes is not None:
+        planes = list(planes)
+        if len(planes) != K:
+            raise Lp2dError(f"planes must hold {K} entries (None where not stored)")
+        parr = (C.c_void_p * K)()
+        for k, pl in enumerate(planes):
+            if pl is None or k == 0:
+                continue
+            _check_like(pl, field, f"planes[{k}]")
+            if pl.data_ptr() in (field.data_ptr(), out.data_ptr()):
+                raise Lp2dError(f"planes[{k}] must not alias the fields")
+            parr[k] = pl.data_ptr()
+    o = optics(area_size, wl, z=z, aperture_radius=aperture_radius, lens_f=lens_f,
+               evanescent=evanescent)
+    with torch.cuda.device(field.device):
+        L.check(lib.lp2d_propagate_stack(n, batch, dt, _ptr(field), _ptr(out), _ptr(screens), K,
+                                         C.byref(o), parr, _stream(field)), "propagate_stack")
     return out
 
 

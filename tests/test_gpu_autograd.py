@@ -207,8 +207,10 @@ def test_checkpoint_stride_policy():
 def test_config5_full_size_checkpointed():
     """8 x 4096^2 complex64, 5 screens (corr. 20 um, 1 rad rms) 2 mm apart, 800 nm,
     area 2 mm; gradient w.r.t. all screens with a memory budget of 3 planes for the
-    stored checkpoints vs. storing every plane: identical gradients (the kernels are
-    deterministic), finite, and a directional derivative that matches a c128 FD."""
+    stored checkpoints vs. storing every plane: the same loss, gradients equal to
+    rounding (a plane input stored by the fused stack's chained row pass and the same
+    plane recomputed by a shorter stack end in different passes, whose last bits can
+    differ), finite, and a directional derivative that matches a c128 FD."""
     from lightprop2d_amd import autograd as AG
     n, area, wl, dz, K, B = 4096, 2e-1, 800e-7, 0.2, 5, 8
     rs = np.random.RandomState(2)
@@ -227,7 +229,7 @@ def test_config5_full_size_checkpointed():
     L1, g1 = run(stride=1)
     L3, g3 = run(budget=3 * plane)
     assert L1 == L3
-    assert torch.equal(g1, g3)
+    assert float((g1 - g3).norm() / g1.norm()) < 1e-6
     assert torch.isfinite(g1).all()
     # directional derivative on the complex128 path
     d = torch.from_numpy(rs.standard_normal((K, n, n)).astype(np.float32)).to(DEV)
@@ -295,3 +297,71 @@ def test_adjoint_workspace_matches_oracle(n, screen):
     work = torch.empty_like(gt)
     gu2, _ = P.propagate_adjoint(gt, ut, z, area, wl, phase=pt, aperture_radius=ap, work=work)
     assert torch.equal(gu2, outs[True][0])
+
+
+@pytest.mark.parametrize("n,dt,B", [(512, torch.complex64, 3), (128, torch.complex64, 2),
+                                    (256, torch.complex128, 2), (1024, torch.complex64, 2),
+                                    (4096, torch.complex64, 1)])
+def test_fused_stack_and_its_planes_vs_oracle(n, dt, B):
+    """lp2d_propagate_stack: K = 4 shared screens with a pupil.  At complex64 three-sweep
+    sizes (512, 1024) the inverse rows, the next screen and the forward rows of
+    consecutive planes run as one chained pass over the block-tiled plane; 128^2 runs
+    the whole-field kernel per plane and complex128 the per-plane sweeps.  The output
+    and the stored plane inputs u_1 .. u_3 against the oracle, field by field; the
+    output equals the plane-by-plane propagation to rounding."""
+    from lightprop2d_amd import propagation as P
+    area, wl, dz, K = 300e-4, 800e-7, 0.02, (3 if n >= 4096 else 4)
+    ap = 0.35 / (area / n)
+    rs = np.random.RandomState(n + B)
+    u0 = (rs.standard_normal((B, n, n)) + 1j * rs.standard_normal((B, n, n)))
+    scr = rs.uniform(0, 2 * np.pi, (K, n, n)).astype(np.float32)
+    tu = torch.from_numpy(u0).to(dt).to(DEV)
+    ts = torch.from_numpy(scr).to(DEV)
+    planes = [None] + [torch.empty_like(tu) for _ in range(K - 1)]
+    print(f"stack n={n} {dt} B={B}", flush=True)
+    out = P.propagate_stack(tu, ts, dz, area, wl, aperture_radius=ap, planes=planes)
+    tol = 1e-5 if dt == torch.complex64 else 1e-10
+    for b in range(B):
+        u = u0[b].astype(np.complex64 if dt == torch.complex64 else np.complex128)
+        for k in range(K):
+            u = oracle.propagate_field(u, area, wl, dz, phase_mask=scr[k].astype(np.float64),
+                                       aperture_radius=ap)
+            got = out[b] if k == K - 1 else planes[k + 1][b]
+            assert rel_l2(got.cpu().numpy(), u) < tol, (b, k)
+    seq = tu
+    for k in range(K):
+        seq = P.propagate(seq, dz, area, wl, phase=ts[k], aperture_radius=ap)
+        got = out if k == K - 1 else planes[k + 1]
+        print(f"  plane {k + 1}: max |fused - per-plane| = "
+              f"{float((got - seq).abs().max()):.3e}", flush=True)
+    assert rel_l2(out.cpu().numpy(), seq.cpu().numpy()) < tol / 10
+    # in place, without stored planes
+    ip = tu.clone()
+    P.propagate_stack(ip, ts, dz, area, wl, aperture_radius=ap, out=ip)
+    assert torch.equal(ip, out)
+
+
+@pytest.mark.parametrize("stride,mode", [(1, "segment"), (2, "segment"), (3, "plane")])
+def test_stack_grad_c64_fused_vs_oracle(stride, mode):
+    """The checkpointed stack in complex64 at 512^2 (forward and recomputation through
+    the fused stack) against the oracle's reverse-mode gradient."""
+    from lightprop2d_amd import autograd as AG
+    n, area, wl, dz, K = 512, 2e-1 / 8, 800e-7, 0.2, 5
+    rs = np.random.RandomState(4)
+    scr = _screens(K, n, area, rs)
+    u0 = (rs.standard_normal((2, n, n)) + 1j * rs.standard_normal((2, n, n)))
+    w = rs.uniform(size=(2, n, n))
+    u_t = torch.from_numpy(u0).to(torch.complex64).to(DEV).requires_grad_(True)
+    s_t = torch.from_numpy(scr).to(DEV).requires_grad_(True)
+    out = AG.propagate_stack(u_t, s_t, dz, area, wl, stride=stride, recompute=mode)
+    L = (torch.from_numpy(w).to(DEV) * out.abs() ** 2).sum()
+    L.backward()
+    gs = np.zeros_like(scr, dtype=np.float64)
+    for b in range(2):
+        u0b = u0[b].astype(np.complex64).astype(np.complex128)
+        uo = oracle.propagate_stack(u0b, scr.astype(np.float64), area, wl, dz)
+        gu, gl = oracle.propagate_stack_grad(u0b, scr.astype(np.float64), area, wl, dz,
+                                             2 * w[b] * uo)
+        gs += np.stack(gl)
+        assert rel_l2(u_t.grad[b].cpu().numpy(), gu) < 2e-5
+    assert rel_l2(s_t.grad.cpu().numpy(), gs) < 2e-5
