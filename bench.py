@@ -150,7 +150,9 @@ def extra_legs(torch, P, L, lib, dev):
     counts one read and one write of every field (16 B per complex64 element), the
     least any propagation can move."""
     out = {"uneven_batch": uneven_leg(torch, P, L, lib, dev),
-           "speckle_chain": speckle_leg(torch, P, dev)}
+           "speckle_chain": speckle_leg(torch, P, dev),
+           "beam2d_chain": beam2d_leg(torch, P, dev),
+           "stack": stack_leg(torch, P, L, lib, dev)}
     out["large_fields"] = large_field_leg(torch, P, L, lib, dev)
     out["generators"] = generator_leg(torch, dev)
     out["resample"] = resample_leg(torch, P, dev)
@@ -257,6 +259,90 @@ def large_field_leg(torch, P, L, lib, dev):
             "sweeps_us": {k: round(ms * 1e3 / c, 1) for k, (c, ms) in sweeps.items()},
             "end_to_end_GBs": round(alg / el / 1e9, 1),
             "end_to_end_frac": round(alg / el / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def stack_leg(torch, P, L, lib, dev):
+    """Configuration 5's forward: 8 fields of 4096^2 complex64 through 5 shared phase
+    screens 2 mm apart (800 nm, area 2 mm), as one fused stack (lp2d_propagate_stack:
+    the inverse rows of a plane, the next screen and the next forward rows in one
+    sweep) against five separate propagations; per-sweep kernel times of the fused
+    stack from the tracer."""
+    n, B, K, area, wl, dz = 4096, 8, 5, 2000e-4, 800e-7, 2000e-4
+    g = torch.Generator(device="cpu").manual_seed(6)
+    u = torch.randn((B, n, n), dtype=torch.complex64, generator=g).to(dev)
+    scr = (torch.randn((K, n, n), generator=g)).to(torch.float32).to(dev)
+    out = torch.empty_like(u)
+    tmp = torch.empty_like(u)
+
+    def fused():
+        P.propagate_stack(u, scr, dz, area, wl, out=out)
+
+    def per_plane():
+        P.propagate(u, dz, area, wl, phase=scr[0], out=tmp)
+        for k in range(1, K):
+            P.propagate(tmp, dz, area, wl, phase=scr[k], out=tmp)
+
+    res = {}
+    for name, fn in (("fused", fused), ("per_plane", per_plane)):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            fn()
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / 5
+        res[name] = {"ms_per_stack": round(el * 1e3, 3), "ms_per_plane": round(el * 1e3 / K, 3),
+                     "plane_GBs": round(16.0 * B * n * n * K / el / 1e9, 1)}
+    L.check(lib.lp2d_trace_enable(1))
+    fused()
+    torch.cuda.synchronize()
+    L.check(lib.lp2d_trace_enable(0))
+    sw = trace_stats(lib, L)
+    res["fused_sweeps_us"] = {k: round(ms * 1e3 / c, 1) for k, (c, ms) in sw.items()}
+    res["speedup"] = round(res["per_plane"]["ms_per_stack"] / res["fused"]["ms_per_stack"], 3)
+    res["workload"] = ("8 x 4096^2 complex64, 5 shared phase screens, 2 mm spacing, 800 nm, "
+                       "area 2 mm: fused stack vs 5 propagations")
+    del u, scr, out, tmp
+    return res
+
+
+def beam2d_leg(torch, P, dev):
+    """Configuration 2 through the drop-in Beam2D: 1024^2 complex64 speckle field,
+    10 x [propagate(50 um), spectral_filter(NA 0.2 pupil array)], then iprofile on the
+    device.  Beam2D keeps the field lazily: each propagate folds its transfer function
+    into the next spectral multiply, each spectral_filter is one elementwise sweep over
+    the spectrum (16 n^2 B with the pupil's 1 n^2 B), and the one inverse FFT runs at
+    the iprofile read.  Reported per Beam2D call (21 calls per chain)."""
+    from lightprop2d_amd import Beam2D
+    from lightprop2d_amd import profiles as PP
+    n, area, wl, NA = 1024, 400e-4, 632e-7, 0.2
+    x = (np.arange(n) - n // 2) * area / n
+    f = PP.random_round_hole_phase(x, x.reshape(-1, 1), 25e-4 * np.sqrt(2),
+                                   rng=np.random.RandomState(0))
+    kx = np.fft.fftfreq(n, area / n)
+    pupil = torch.from_numpy((kx ** 2 + kx.reshape(-1, 1) ** 2) <= (NA / wl) ** 2).to(dev)
+    b = Beam2D(area, n, wl, init_field=f, complex_bits=64, numpy_output=False)
+
+    def chain():
+        for _ in range(10):
+            b.propagate(50e-4)
+            b.spectral_filter(f_init=pupil)
+        return b.iprofile
+
+    chain()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 10
+    for _ in range(reps):
+        chain()
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / reps
+    return {"workload": "Beam2D 1024^2 complex64: 10 x [propagate 50 um, spectral_filter(NA 0.2 "
+                        "pupil)] + iprofile on the device",
+            "ms_per_chain": round(el * 1e3, 4), "value": round(21 / el, 1),
+            "unit": "Beam2D calls/s",
+            "bytes_per_filter_call": 17 * n * n,
+            "sweeps_per_chain": "10 elementwise + 2 (one inverse FFT) + 1 (intensity)"}
 
 
 MFMA_FP64_PEAK_TFS = 78.6  # MI355X dense FP64 matrix peak
@@ -388,6 +474,8 @@ def generator_leg(torch, dev):
 LEGS = {
     "uneven_batch": lambda torch, P, L, lib, dev: uneven_leg(torch, P, L, lib, dev),
     "speckle_chain": lambda torch, P, L, lib, dev: speckle_leg(torch, P, dev),
+    "beam2d_chain": lambda torch, P, L, lib, dev: beam2d_leg(torch, P, dev),
+    "stack": lambda torch, P, L, lib, dev: stack_leg(torch, P, L, lib, dev),
     "large_fields": lambda torch, P, L, lib, dev: large_field_leg(torch, P, L, lib, dev),
     "resample": lambda torch, P, L, lib, dev: resample_leg(torch, P, dev),
     "modes": lambda torch, P, L, lib, dev: modes_leg(torch, P, dev),

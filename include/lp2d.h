@@ -340,8 +340,9 @@ int lp2d_trace_collect(lp2d_trace_stat* stats, int max_stats, int* n_stats);
 /* ---------------------------------------------------------------- tuning */
 /* Number of kernel variants (schedules) available for (n, dtype, kind), kind 0 =
  * row pass, 1 = column pass, 2 = whole-field-in-LDS kernel (0 or 1: sizes that use
- * it run every operation in one launch); and the one in use.  Used by the
- * autotuner. */
+ * it run every operation in one launch), 3 = chained-plane row pass of
+ * lp2d_propagate_stack (0 where the plane is never block-tiled); and the one in use
+ * (kinds 0, 1, 3).  Used by the autotuner. */
 int lp2d_variant_count(int n, int dtype, int kind);
 int lp2d_set_variant(int n, int dtype, int kind, int variant);
 int lp2d_get_variant(int n, int dtype, int kind);

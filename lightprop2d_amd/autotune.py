@@ -1,7 +1,7 @@
-"""Autotuner: picks the row-pass and column-pass schedule (elements per thread E,
-rows per workgroup, columns per strip W) for each grid size and precision by timing
-every registered variant on the device, then installs the winners with
-``lp2d_set_variant``.
+"""Autotuner: picks the row-pass, column-pass and chained-plane row-pass schedule
+(elements per thread E, rows per workgroup, columns per strip W) for each grid size
+and precision by timing every registered variant on the device, then installs the
+winners with ``lp2d_set_variant``.
 
 Results are cached as JSON keyed by "<gfx arch>:<dtype>:<n>".  A table measured on
 MI355X ships with the package (tuned_gfx950.json) and is applied on import of
@@ -40,7 +40,7 @@ def variant_names(n: int, dtype: str = "c64"):
     lib = L.lib()
     buf = C.create_string_buffer(128)
     out = {}
-    for kind, label in ((0, "rows"), (1, "cols")):
+    for kind, label in ((0, "rows"), (1, "cols"), (3, "chain")):
         names = []
         for v in range(lib.lp2d_variant_count(n, _DT[dtype], kind)):
             lib.lp2d_variant_name(n, _DT[dtype], kind, v, buf, 128)
@@ -113,14 +113,34 @@ def tune(n: int, dtype: str = "c64", batch: int | None = None, reps: int = 5,
             best_c, best_ct = v, ct
     L.check(lib.lp2d_set_variant(n, dt, 1, best_c))
     result["best"] = {"rows": best_r, "cols": best_c}
+    entry = {"rows": best_r, "cols": best_c, "rows_name": names["rows"][best_r],
+             "cols_name": names["cols"][best_c], "rows_ms": best_rt, "cols_ms": best_ct,
+             "batch": batch, "time": time.time()}
+    # chained-plane row pass of lp2d_propagate_stack, timed on a 3-screen stack
+    nch = lib.lp2d_variant_count(n, dt, 3)
+    if nch > 0:
+        scr = torch.rand((3, n, n), device="cuda", dtype=torch.float32)
+
+        def stack_step():
+            P.propagate_stack(u, scr, 0.01, 400e-4, 632e-7, out=out)
+
+        result["chain"] = []
+        best_h, best_ht = 0, float("inf")
+        for v in range(nch):
+            L.check(lib.lp2d_set_variant(n, dt, 3, v))
+            t = _time_variant(lib, stack_step, reps)
+            ht = sum(ms for k, ms in t.items() if k.startswith("rows_chain"))
+            result["chain"].append({"variant": v, "name": names["chain"][v], "ms": ht})
+            if ht < best_ht:
+                best_h, best_ht = v, ht
+        L.check(lib.lp2d_set_variant(n, dt, 3, best_h))
+        result["best"]["chain"] = best_h
+        entry.update({"chain": best_h, "chain_name": names["chain"][best_h],
+                      "chain_ms": best_ht})
     if verbose:
         print(json.dumps(result))
     if save:
-        _save({f"{arch()}:{dtype}:{n}": {"rows": best_r, "cols": best_c,
-                                          "rows_name": names["rows"][best_r],
-                                          "cols_name": names["cols"][best_c],
-                                          "rows_ms": best_rt, "cols_ms": best_ct,
-                                          "batch": batch, "time": time.time()}})
+        _save({f"{arch()}:{dtype}:{n}": entry})
     return result
 
 
@@ -150,7 +170,9 @@ def apply(table: dict, gfx: str | None = None) -> int:
             continue
         n = int(n)
         names = variant_names(n, dtype)
-        for kind, name in ((0, "rows"), (1, "cols")):
+        for kind, name in ((0, "rows"), (1, "cols"), (3, "chain")):
+            if name not in v and name + "_name" not in v:
+                continue  # (tables without a chained-pass entry keep its default)
             want = v.get(name + "_name")
             if want is not None:  # match by schedule, not by index
                 if want in names[name]:

@@ -69,7 +69,7 @@ DeviceTables g_dev[kMaxDevices];
 
 Registry<float>* g_rf = nullptr;
 Registry<double>* g_rd = nullptr;
-int g_sel[2][2][kLogMax + 1];  // [dtype][kind][log2 n] -> selected variant
+int g_sel[2][3][kLogMax + 1];  // [dtype][kind: rows, cols, chain][log2 n] -> variant
 int g_chunk[2][kLogMax + 1];   // fields per three-sweep chunk (0 = whole batch)
 
 void ensure_registry() {
@@ -720,8 +720,9 @@ int op_stack(int n, int lg, int batch, const void* in, void* out, const float* s
   if (o->aperture_radius > 0.0) mid |= F_APERTURE;
   a.flags = mid;
   const int tl = use_small<T>(lg) ? 0 : tiled_flag<T>(a, lg);
-  const RowVar<T>& rv = reg<T>()->rows[lg][g_sel[dt<T>()][0][lg]];
-  if (!tl || !rv.chain_fn || K < 2 || o->lens_f != 0.0) {
+  const int cv = g_sel[dt<T>()][2][lg];
+  const RowChainLaunch<T> chain = reg<T>()->nchain[lg] > cv ? reg<T>()->chain[lg][cv] : nullptr;
+  if (!tl || !chain || K < 2 || o->lens_f != 0.0) {
     const void* cur = in;
     for (int k = 0; k < K; ++k) {
       void* dst = (k + 1 < K && plane(k + 1)) ? plane(k + 1) : out;
@@ -761,13 +762,13 @@ int op_stack(int n, int lg, int batch, const void* in, void* out, const float* s
     std::string nm;
     if (g_trace_on) {
       char b[64];
-      snprintf(b, sizeof(b), "rows_chain n=%d %s E=%d", n, tname<T>(), rv.E);
+      snprintf(b, sizeof(b), "rows_chain n=%d %s E=%d", n, tname<T>(), reg<T>()->chain_E[lg][cv]);
       nm = b;
     }
     hipError_t e;
     {
       TraceScope ts(s, nm);
-      e = rv.chain_fn(r, s);
+      e = chain(r, s);
     }
     if (e != hipSuccess) return hip_fail(e, "chained row pass launch");
   }
@@ -1226,18 +1227,20 @@ int lp2d_sum_planes(int64_t count, int batch, const float* in, float* out, void*
 int lp2d_variant_count(int n, int dtype, int kind) {
   ensure_registry();
   const int lg = log2_exact(n);
-  if (lg < 1 || lg > kLogMax || (dtype != 0 && dtype != 1) || kind < 0 || kind > 2) return 0;
+  if (lg < 1 || lg > kLogMax || (dtype != 0 && dtype != 1) || kind < 0 || kind > 3) return 0;
   if (kind == 2) return dtype == 0 ? (int)g_rf->has_small[lg] : (int)g_rd->has_small[lg];
+  if (kind == 3) return dtype == 0 ? g_rf->nchain[lg] : g_rd->nchain[lg];
   if (dtype == 0) return kind == 0 ? g_rf->nrows[lg] : g_rf->ncols[lg];
   return kind == 0 ? g_rd->nrows[lg] : g_rd->ncols[lg];
 }
 
 int lp2d_set_variant(int n, int dtype, int kind, int variant) {
-  if (kind != 0 && kind != 1) return fail(LP2D_E_ARG, "kind must be 0 (rows) or 1 (columns)");
+  if (kind != 0 && kind != 1 && kind != 3)
+    return fail(LP2D_E_ARG, "kind must be 0 (rows), 1 (columns) or 3 (chained rows)");
   const int cnt = lp2d_variant_count(n, dtype, kind);
   if (cnt == 0) return fail(LP2D_E_SIZE, "no variants for n=%d dtype=%d kind=%d", n, dtype, kind);
   if (variant < 0 || variant >= cnt) return fail(LP2D_E_ARG, "variant %d out of range [0,%d)", variant, cnt);
-  g_sel[dtype][kind][log2_exact(n)] = variant;
+  g_sel[dtype][kind == 3 ? 2 : kind][log2_exact(n)] = variant;
   return LP2D_OK;
 }
 
@@ -1258,9 +1261,9 @@ int lp2d_get_chunk(int n, int dtype) {
 }
 
 int lp2d_get_variant(int n, int dtype, int kind) {
-  if ((kind != 0 && kind != 1) || lp2d_variant_count(n, dtype, kind) == 0)
+  if ((kind != 0 && kind != 1 && kind != 3) || lp2d_variant_count(n, dtype, kind) == 0)
     return fail(LP2D_E_SIZE, "no variants");
-  return g_sel[dtype][kind][log2_exact(n)];
+  return g_sel[dtype][kind == 3 ? 2 : kind][log2_exact(n)];
 }
 
 int lp2d_variant_name(int n, int dtype, int kind, int variant, char* buf, int buflen) {
@@ -1268,6 +1271,13 @@ int lp2d_variant_name(int n, int dtype, int kind, int variant, char* buf, int bu
   if (kind == 2 || variant < 0 || variant >= cnt || !buf || buflen < 1)
     return fail(LP2D_E_ARG, "bad variant query");
   const int lg = log2_exact(n);
+  if (kind == 3) {
+    const Registry<float>* rf = g_rf;
+    const Registry<double>* rd = g_rd;
+    const int E = dtype ? rd->chain_E[lg][variant] : rf->chain_E[lg][variant];
+    const int L = dtype ? rd->chain_lines[lg][variant] : rf->chain_lines[lg][variant];
+    return snprintf(buf, buflen, "chain E=%d lines=%d threads=%d", E, L, (n / E) * L);
+  }
   if (kind == 0) {
     const int E = dtype ? g_rd->rows[lg][variant].E : g_rf->rows[lg][variant].E;
     const int L = dtype ? g_rd->rows[lg][variant].lines : g_rf->rows[lg][variant].lines;

@@ -74,8 +74,10 @@ __global__ __launch_bounds__((N / E) * LINES) LP2D_OCC void k_rows_grad(PassArgs
 }
 
 // Chained-plane row pass: one workgroup per row tile of LINES rows of one field.
-template <typename T, int N, int E, int LINES, int PS>
-__global__ __launch_bounds__((N / E) * LINES) LP2D_OCC void k_rows_chain(PassArgs<T> a) {
+// WPE: waves per SIMD the register budget is sized for.
+template <typename T, int N, int E, int LINES, int PS, int WPE>
+__global__ __launch_bounds__((N / E) * LINES) __attribute__((amdgpu_waves_per_eu(WPE)))
+void k_rows_chain(PassArgs<T> a) {
   __shared__ cx<T> lds[LINES * RowPitch<N, PS, LINES>::value];
   constexpr int TPF = N / LINES;
   const int tile = blockIdx.x;
@@ -279,13 +281,13 @@ template <typename T, int N, int E, int LINES, int PS>
 constexpr bool chain_ok() {
   return sizeof(T) == 4 && N % LINES == 0 && TiledRowMap<N, N / E, LINES>::ok;
 }
-template <typename T, int N, int E, int LINES, int PS>
+template <typename T, int N, int E, int LINES, int PS, int WPE = 2>
 hipError_t launch_rows_chain(const PassArgs<T>& a, hipStream_t s) {
   if constexpr (chain_ok<T, N, E, LINES, PS>()) {
     if (!(a.flags & F_TILED) || !a.phase || a.src_tab || a.dst_tab) return hipErrorInvalidValue;
     constexpr int threads = (N / E) * LINES;
     const unsigned grid = (unsigned)((long long)a.batch * (N / LINES));
-    k_rows_chain<T, N, E, LINES, PS><<<grid, threads, 0, s>>>(a);
+    k_rows_chain<T, N, E, LINES, PS, WPE><<<grid, threads, 0, s>>>(a);
     return hipGetLastError();
   } else {
     (void)a;
@@ -451,10 +453,7 @@ void add_row(Registry<T>& r) {
   // threads, measured 1156 us against 749 us for one row per workgroup)
   r.rows[lg][k++] = RowVar<T>{&launch_rows<T, N, E, S::LINES, S::PS>,
                               &launch_rows_grad<T, N, E, S::LINES, S::PS, (E <= 8)>, E,
-                              S::LINES, S::THREADS, S::LDS, S::LINES, 0,
-                              chain_ok<T, N, E, S::LINES, S::PS>()
-                                  ? &launch_rows_chain<T, N, E, S::LINES, S::PS>
-                                  : nullptr};
+                              S::LINES, S::THREADS, S::LDS, S::LINES, 0};
 }
 
 template <typename T, int N, int E>
@@ -467,10 +466,7 @@ void add_row_pipe(Registry<T>& r) {
   if constexpr (lds <= kLdsBudget)
     r.rows[lg][k++] = RowVar<T>{&launch_rows_pipe<T, N, E, S::LINES, S::PS>,
                                 &launch_rows_grad<T, N, E, S::LINES, S::PS, (E <= 8)>, E,
-                                S::LINES, S::THREADS, lds, S::LINES, 2,
-                                chain_ok<T, N, E, S::LINES, S::PS>()
-                                    ? &launch_rows_chain<T, N, E, S::LINES, S::PS>
-                                    : nullptr};
+                                S::LINES, S::THREADS, lds, S::LINES, 2};
 }
 
 template <typename T, int N, int E, int WT, bool HOIST = false>
@@ -532,6 +528,42 @@ void register_n(Registry<T>& r) {
     } else {
       add_row<T, N, 16>(r);
       add_col<T, N, 16, 8>(r);
+    }
+  }
+  // chained-plane row pass: 32 elements per thread in the row pass's shape (2 waves
+  // per SIMD), and 16 elements per thread in row tiles of at least one tile height
+  // (4 waves per SIMD).  Variant 0 is the faster one measured per size (stack of 5
+  // screens, per pass): 4096 x8: 683 us (32) vs 851 us (16); 1024 x32: 136 vs 150;
+  // 512 x64: 72.9 vs 69.6; 256 x128: 35.2 vs 31.5.
+  r.nchain[lg] = 0;
+  if constexpr (F32 && N >= 256) {
+    using S = RowShape<T, N, 32>;
+    constexpr int L16 = cmax(tile_rows(N), 256 / (N / 16));
+    constexpr int PS16 = cmax(1, ilog2c(16));
+    constexpr bool ok32 = chain_ok<T, N, 32, S::LINES, S::PS>();
+    constexpr bool ok16 = (N / 16) * L16 <= 1024 && chain_ok<T, N, 16, L16, PS16>();
+    auto add32 = [&] {
+      if constexpr (ok32) {
+        const int k = r.nchain[lg]++;
+        r.chain[lg][k] = &launch_rows_chain<T, N, 32, S::LINES, S::PS, 2>;
+        r.chain_E[lg][k] = 32;
+        r.chain_lines[lg][k] = S::LINES;
+      }
+    };
+    auto add16 = [&] {
+      if constexpr (ok16) {
+        const int k = r.nchain[lg]++;
+        r.chain[lg][k] = &launch_rows_chain<T, N, 16, L16, PS16, 4>;
+        r.chain_E[lg][k] = 16;
+        r.chain_lines[lg][k] = L16;
+      }
+    };
+    if constexpr (N >= 1024) {
+      add32();
+      add16();
+    } else {
+      add16();
+      add32();
     }
   }
   using SS = SmallShape<T, N, EMAX>;

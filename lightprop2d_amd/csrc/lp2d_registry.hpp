@@ -28,7 +28,6 @@ struct RowVar {
   int E, lines, threads, lds;
   int grad_lines;  // row tile of grad_fn: whole 8-row blocks where the threads allow
   int kind;        // bit 1: software-pipelined over the row tiles of the batch
-  RowChainLaunch<T> chain_fn;  // inverse rows + next screen + forward rows (tiled)
 };
 template <typename T>
 struct ColVar {
@@ -54,6 +53,11 @@ struct Registry {
   int ncols[kLogMax + 1];
   SmallVar<T> small[kLogMax + 1];
   bool has_small[kLogMax + 1];
+  // chained-plane row pass (inverse rows + next screen + forward rows over the
+  // block-tiled plane): its shapes per size (none where the plane is never tiled)
+  RowChainLaunch<T> chain[kLogMax + 1][2];
+  int chain_E[kLogMax + 1][2], chain_lines[kLogMax + 1][2];
+  int nchain[kLogMax + 1];
 };
 
 // Workgroups of a column-pass kernel resident at once on the current device.

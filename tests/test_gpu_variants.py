@@ -137,3 +137,69 @@ def test_pipelined_groups_at_large_and_single_batches(n, batch):
         print(f"n={n} batch={batch} field={b}", flush=True)
         exp = oracle.propagate_field(u[b].cpu().numpy(), area, wl, z, aperture_radius=na / wl)
         assert rel_l2(out[b].cpu().numpy(), exp) < 1e-5
+
+
+@pytest.mark.parametrize("n", [256, 512, 1024, 4096])
+def test_every_chain_variant_matches_oracle(n):
+    """Every chained-plane row pass (kind 3) of lp2d_propagate_stack: a 3-screen stack
+    with a pupil against the oracle for one field, the stored middle planes included,
+    and equal to the plane-by-plane propagation to rounding."""
+    from lightprop2d_amd import propagation as P
+    lib = _lib()
+    area, wl, dz, K = 300e-4, 800e-7, 0.02, 3
+    ap = 0.3 / (area / n)
+    B = 1 if n >= 4096 else 3
+    rs = np.random.RandomState(n + 7)
+    u = (rs.standard_normal((B, n, n)) + 1j * rs.standard_normal((B, n, n))).astype(np.complex64)
+    scr = rs.uniform(0, 2 * np.pi, (K, n, n)).astype(np.float32)
+    exp = [u[B - 1].astype(np.complex128)]
+    for k in range(K):
+        exp.append(oracle.propagate_field(exp[-1], area, wl, dz,
+                                          phase_mask=scr[k].astype(np.float64),
+                                          aperture_radius=ap))
+    tu = torch.from_numpy(u).to(DEV)
+    ts = torch.from_numpy(scr).to(DEV)
+    seq = tu
+    for k in range(K):
+        seq = P.propagate(seq, dz, area, wl, phase=ts[k], aperture_radius=ap)
+    names = _names(lib, n, 0, 3)
+    assert len(names) >= 1
+    default = lib.lp2d_get_variant(n, 0, 3)
+    try:
+        for v, name in enumerate(names):
+            print(f"chain n={n} variant={v} [{name}]", flush=True)
+            assert lib.lp2d_set_variant(n, 0, 3, v) == 0
+            planes = [None] + [torch.empty_like(tu) for _ in range(K - 1)]
+            out = P.propagate_stack(tu, ts, dz, area, wl, aperture_radius=ap, planes=planes)
+            for k in range(1, K + 1):
+                got = out if k == K else planes[k]
+                assert rel_l2(got[B - 1].cpu().numpy(), exp[k]) < 1e-5, (name, k)
+            assert rel_l2(out.cpu().numpy(), seq.cpu().numpy()) < 1e-6, name
+    finally:
+        lib.lp2d_set_variant(n, 0, 3, default)
+
+
+def test_autotuner_runs_and_installs_a_measured_choice():
+    """autotune.tune at 256^2 complex64: every row, column and chained-pass variant is
+    timed, the fastest is installed, and the propagation still matches the oracle."""
+    from lightprop2d_amd import autotune
+    from lightprop2d_amd import propagation as P
+    lib = _lib()
+    n = 256
+    saved = {k: lib.lp2d_get_variant(n, 0, k) for k in (0, 1, 3)}
+    try:
+        res = autotune.tune(n, "c64", batch=4, reps=1, save=False)
+        names = autotune.variant_names(n, "c64")
+        assert len(res["rows"]) == len(names["rows"]) and len(res["cols"]) == len(names["cols"])
+        assert len(res["chain"]) == len(names["chain"]) >= 1
+        for kind, key in ((0, "rows"), (1, "cols"), (3, "chain")):
+            best = min(range(len(res[key])), key=lambda i: res[key][i]["ms"])
+            assert lib.lp2d_get_variant(n, 0, kind) == res["best"][key] == best
+        rs = np.random.RandomState(21)
+        u = (rs.standard_normal((2, n, n)) + 1j * rs.standard_normal((2, n, n))).astype(np.complex64)
+        area, wl, z = 200e-4, 632e-7, 30e-4
+        out = P.propagate(torch.from_numpy(u).to(DEV), z, area, wl)
+        assert rel_l2(out[1].cpu().numpy(), oracle.propagate_field(u[1], area, wl, z)) < 1e-5
+    finally:
+        for k, v in saved.items():
+            lib.lp2d_set_variant(n, 0, k, v)
