@@ -50,7 +50,7 @@ def _lib():
 
 def _small_path(n: int, dtype: str = "c64") -> bool:
     """True when size n runs every operation in one whole-field-in-LDS launch."""
-    return L.lib().lp2d_variant_count(n, L.LP2D_C64 if dtype == "c64" else L.LP2D_C128, 2) == 1
+    return L.lib().lp2d_variant_count(n, L.LP2D_C64 if dtype == "c64" else L.LP2D_C128, 2) >= 1
 
 
 def _stream(t: torch.Tensor):
