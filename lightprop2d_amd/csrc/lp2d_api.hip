@@ -1276,7 +1276,9 @@ int lp2d_variant_name(int n, int dtype, int kind, int variant, char* buf, int bu
     const Registry<double>* rd = g_rd;
     const int E = dtype ? rd->chain_E[lg][variant] : rf->chain_E[lg][variant];
     const int L = dtype ? rd->chain_lines[lg][variant] : rf->chain_lines[lg][variant];
-    return snprintf(buf, buflen, "chain E=%d lines=%d threads=%d", E, L, (n / E) * L);
+    const int tl = dtype ? rd->chain_twl[lg][variant] : rf->chain_twl[lg][variant];
+    return snprintf(buf, buflen, "chain E=%d lines=%d threads=%d%s", E, L, (n / E) * L,
+                    tl ? " lds-twiddles" : "");
   }
   if (kind == 0) {
     const int E = dtype ? g_rd->rows[lg][variant].E : g_rf->rows[lg][variant].E;

@@ -82,7 +82,24 @@ void k_rows_chain(PassArgs<T> a) {
   constexpr int TPF = N / LINES;
   const int tile = blockIdx.x;
   const int b = tile / TPF;
-  rows_chain_body<T, N, E, LINES, PS>(a, b, (tile - b * TPF) * LINES, lds, threadIdx.x);
+  rows_chain_body<T, N, E, LINES, PS>(a, b, (tile - b * TPF) * LINES, lds, nullptr,
+                                      threadIdx.x);
+}
+
+// The same with the twiddles staged once in LDS (compact last pass when CL), each
+// workgroup taking every gridDim.x-th row tile of the batch (one resident round).
+template <typename T, int N, int E, int LINES, int PS, int WPE, bool CL>
+__global__ __launch_bounds__((N / E) * LINES) __attribute__((amdgpu_waves_per_eu(WPE)))
+void k_rows_chain_l(PassArgs<T> a, int nunits) {
+  __shared__ cx<T> lds[LINES * RowPitch<N, PS, LINES>::value];
+  __shared__ cx<T> twl[TwlLen<N, E, CL>::value];
+  twl_fill<T, N, E, (N / E) * LINES, 1, CL>(twl, a.tw, threadIdx.x);
+  __syncthreads();
+  constexpr int TPF = N / LINES;
+  for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const int b = u / TPF;
+    rows_chain_body<T, N, E, LINES, PS, CL>(a, b, (u - b * TPF) * LINES, lds, twl, threadIdx.x);
+  }
 }
 
 // Grid: (N/W strips) x (field groups of `per` fields).
@@ -288,6 +305,31 @@ hipError_t launch_rows_chain(const PassArgs<T>& a, hipStream_t s) {
     constexpr int threads = (N / E) * LINES;
     const unsigned grid = (unsigned)((long long)a.batch * (N / LINES));
     k_rows_chain<T, N, E, LINES, PS, WPE><<<grid, threads, 0, s>>>(a);
+    return hipGetLastError();
+  } else {
+    (void)a;
+    (void)s;
+    return hipErrorInvalidValue;
+  }
+}
+
+template <typename T, int N, int E, int LINES, bool CL>
+constexpr int chain_l_lds() {
+  return (LINES * RowPitch<N, cmax(1, ilog2c(E)), LINES>::value + TwlLen<N, E, CL>::value) *
+         (int)sizeof(cx<T>);
+}
+template <typename T, int N, int E, int LINES, int PS, int WPE, bool CL>
+hipError_t launch_rows_chain_l(const PassArgs<T>& a, hipStream_t s) {
+  if constexpr (chain_ok<T, N, E, LINES, PS>() && chain_l_lds<T, N, E, LINES, CL>() <= kLdsBudget) {
+    if (!(a.flags & F_TILED) || !a.phase || a.src_tab || a.dst_tab) return hipErrorInvalidValue;
+    constexpr int threads = (N / E) * LINES;
+    const long long units = (long long)a.batch * (N / LINES);
+    if (units > 0x7fffffff) return hipErrorInvalidValue;
+    static std::atomic<int> slot{0};
+    const int slots = cached_slots(slot, (const void*)&k_rows_chain_l<T, N, E, LINES, PS, WPE, CL>,
+                                   threads);
+    const int grid = (int)(units < slots ? units : slots);
+    k_rows_chain_l<T, N, E, LINES, PS, WPE, CL><<<grid, threads, 0, s>>>(a, (int)units);
     return hipGetLastError();
   } else {
     (void)a;
@@ -531,11 +573,13 @@ void register_n(Registry<T>& r) {
     }
   }
   // chained-plane row pass: 32 elements per thread in the row pass's shape (2 waves
-  // per SIMD), and 16 elements per thread in row tiles of at least one tile height
-  // (4 waves per SIMD).  Variant 0 is the faster one measured per size (stack of 5
-  // screens, per pass): 4096 x8: 683 us (32) vs 851 us (16); 1024 x32: 136 vs 150;
-  // 512 x64: 72.9 vs 69.6; 256 x128: 35.2 vs 31.5.
+  // per SIMD) with global or LDS-staged twiddles, and 16 elements per thread in row
+  // tiles of at least one tile height (4 waves per SIMD).  Variant 0 is the fastest
+  // measured per size (stack of 5 screens, per pass; 32 global / 32 LDS / 16):
+  // 4096 x8: 686 / 627 / 846 us; 1024 x32: 142 / 117 / 148; 512 x64: 67 / 63 / 81;
+  // 256 x128: 36 / 34 / 33.
   r.nchain[lg] = 0;
+  for (int k = 0; k < 4; ++k) r.chain_twl[lg][k] = 0;
   if constexpr (F32 && N >= 256) {
     using S = RowShape<T, N, 32>;
     constexpr int L16 = cmax(tile_rows(N), 256 / (N / 16));
@@ -558,12 +602,30 @@ void register_n(Registry<T>& r) {
         r.chain_lines[lg][k] = L16;
       }
     };
-    if constexpr (N >= 1024) {
+    // twiddles in LDS: the 32-element shape (rows of the row pass, or 4 rows at 4096
+    // with the compact last-pass table: one 512-thread workgroup per CU)
+    constexpr int LL = N >= 4096 ? 4 : S::LINES;
+    constexpr bool CLL = N >= 4096;
+    constexpr bool okl = chain_ok<T, N, 32, LL, S::PS>() &&
+                         chain_l_lds<T, N, 32, LL, CLL>() <= kLdsBudget &&
+                         (N / 32) * LL <= 1024;
+    auto add32l = [&] {
+      if constexpr (okl) {
+        const int k = r.nchain[lg]++;
+        r.chain[lg][k] = &launch_rows_chain_l<T, N, 32, LL, S::PS, 2, CLL>;
+        r.chain_E[lg][k] = 32;
+        r.chain_lines[lg][k] = LL;
+        r.chain_twl[lg][k] = 1;
+      }
+    };
+    if constexpr (N >= 512) {
+      add32l();
       add32();
       add16();
     } else {
       add16();
       add32();
+      add32l();
     }
   }
   using SS = SmallShape<T, N, EMAX>;

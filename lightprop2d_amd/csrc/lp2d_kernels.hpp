@@ -629,9 +629,11 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
 // store the tile in place.  The thread map (TiledRowMap) makes every tiled access
 // cover whole tiles or 256-byte runs and every natural-order store whole 128-byte
 // lines.  Three sweeps per plane become two.
-template <typename T, int N, int E, int LINES, int PS>
+// twl: null for the twiddles of the global table; otherwise the per-pass LDS tables of
+// twl_fill (compact last pass when CL).
+template <typename T, int N, int E, int LINES, int PS, bool CL = false>
 __device__ __forceinline__ void rows_chain_body(const PassArgs<T>& a, int b, int y0,
-                                                cx<T>* lds, int tid) {
+                                                cx<T>* lds, const cx<T>* twl, int tid) {
   constexpr int TL = N / E;
   constexpr int L = RowPitch<N, PS, LINES>::value;
   constexpr int TR = tile_rows(N), TC = tile_cols(N);
@@ -648,7 +650,10 @@ __device__ __forceinline__ void rows_chain_body(const PassArgs<T>& a, int b, int
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = gld<NT_ROW_LD>(src + (lo_t + TS * (unsigned)m));
   LineAcc<T, PS> acc{lds + l * L};
-  line_fft<T, N, E, true>(v, t, acc, a.tw_inv + (N - 2));
+  if (twl)
+    line_fft_l<T, N, E, true, CL>(v, t, acc, twl);
+  else
+    line_fft<T, N, E, true>(v, t, acc, a.tw_inv + (N - 2));
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = cscale(v[m], a.scale);
   if (a.flags & F_FIELD_OUT) {
@@ -658,7 +663,10 @@ __device__ __forceinline__ void rows_chain_body(const PassArgs<T>& a, int b, int
     for (int m = 0; m < E; ++m) gst<NT_ROWI_ST>(ck + (t + TL * m), v[m]);
   }
   phase_mul_all<T, E, false>(a, v, b, y, t, TL, N);
-  line_fft<T, N, E, false>(v, t, acc, a.tw + (N - 2));
+  if (twl)
+    line_fft_l<T, N, E, false, CL>(v, t, acc, twl);
+  else
+    line_fft<T, N, E, false>(v, t, acc, a.tw + (N - 2));
   LP2D_GLB cx<T>* __restrict__ dst = field_dst(a, b) + ubase;
 #pragma unroll
   for (int m = 0; m < E; ++m) gst<NT_ROW_ST>(dst + (lo_t + TS * (unsigned)m), v[m]);

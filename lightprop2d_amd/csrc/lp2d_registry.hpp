@@ -55,8 +55,9 @@ struct Registry {
   bool has_small[kLogMax + 1];
   // chained-plane row pass (inverse rows + next screen + forward rows over the
   // block-tiled plane): its shapes per size (none where the plane is never tiled)
-  RowChainLaunch<T> chain[kLogMax + 1][2];
-  int chain_E[kLogMax + 1][2], chain_lines[kLogMax + 1][2];
+  RowChainLaunch<T> chain[kLogMax + 1][4];
+  int chain_E[kLogMax + 1][4], chain_lines[kLogMax + 1][4];
+  int chain_twl[kLogMax + 1][4];  // 1: twiddles staged in LDS
   int nchain[kLogMax + 1];
 };
 

@@ -51,8 +51,9 @@ def test_host_only_queries(lib):
     assert lib.lp2d_set_variant(1024, 0, 1, 99) == -1
     # chained-plane row passes (kind 3): complex64 sizes whose plane is block-tiled
     for n in (256, 512, 1024, 4096):
-        assert lib.lp2d_variant_count(n, 0, 3) == 2
-        for v in range(2):
+        cnt = lib.lp2d_variant_count(n, 0, 3)
+        assert 2 <= cnt <= 4
+        for v in range(cnt):
             assert lib.lp2d_variant_name(n, 0, 3, v, buf, 128) > 0
             assert buf.value.startswith(b"chain E=")
     assert lib.lp2d_variant_count(1024, 1, 3) == 0
