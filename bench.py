@@ -265,8 +265,10 @@ def stack_leg(torch, P, L, lib, dev):
     """Configuration 5's forward: 8 fields of 4096^2 complex64 through 5 shared phase
     screens 2 mm apart (800 nm, area 2 mm), as one fused stack (lp2d_propagate_stack:
     the inverse rows of a plane, the next screen and the next forward rows in one
-    sweep) against five separate propagations; per-sweep kernel times of the fused
-    stack from the tracer."""
+    sweep) against five separate propagations; its adjoint as one fused stack adjoint
+    (lp2d_propagate_stack_adjoint) against five separate adjoints; one checkpointed
+    gradient step through torch.autograd; per-sweep kernel times of the fused forms
+    from the tracer."""
     n, B, K, area, wl, dz = 4096, 8, 5, 2000e-4, 800e-7, 2000e-4
     g = torch.Generator(device="cpu").manual_seed(6)
     u = torch.randn((B, n, n), dtype=torch.complex64, generator=g).to(dev)
@@ -300,9 +302,66 @@ def stack_leg(torch, P, L, lib, dev):
     sw = trace_stats(lib, L)
     res["fused_sweeps_us"] = {k: round(ms * 1e3 / c, 1) for k, (c, ms) in sw.items()}
     res["speedup"] = round(res["per_plane"]["ms_per_stack"] / res["fused"]["ms_per_stack"], 3)
+    # the adjoint, given the plane inputs: one fused stack adjoint (2K + 1 sweeps, screen
+    # gradients summed over the batch) against K separate adjoints
+    planes = [u] + [torch.empty_like(u) for _ in range(K - 1)]
+    P.propagate_stack(u, scr, dz, area, wl, out=out, planes=[None] + planes[1:])
+    gout = torch.randn((B, n, n), dtype=torch.complex64, generator=g).to(dev)
+    gin = torch.empty_like(u)
+    gs = torch.empty((K, n, n), dtype=torch.float32, device=dev)
+
+    def fused_adj():
+        P.propagate_stack_adjoint(gout, planes, scr, dz, area, wl, out=gin, grad_screens=gs)
+
+    def per_plane_adj():
+        gg = gout
+        for k in range(K - 1, -1, -1):
+            gg, gk = P.propagate_adjoint(gg, planes[k], dz, area, wl, phase=scr[k])
+            gs[k].copy_(gk)
+
+    adj = {}
+    for name, fn in (("fused", fused_adj), ("per_plane", per_plane_adj)):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            fn()
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / 5
+        adj[name] = {"ms_per_stack": round(el * 1e3, 3), "ms_per_plane": round(el * 1e3 / K, 3)}
+    L.check(lib.lp2d_trace_enable(1))
+    fused_adj()
+    torch.cuda.synchronize()
+    L.check(lib.lp2d_trace_enable(0))
+    sw = trace_stats(lib, L)
+    adj["fused_sweeps_us"] = {k: round(ms * 1e3 / c, 1) for k, (c, ms) in sw.items()}
+    adj["speedup"] = round(adj["per_plane"]["ms_per_stack"] / adj["fused"]["ms_per_stack"], 3)
+    res["adjoint"] = adj
+    del planes, gout, gin, gs
+    # one gradient step of the checkpointed stack (every plane input kept): fused
+    # forward storing the planes, fused adjoint with the screen gradients
+    from lightprop2d_amd import autograd as AG
+    ur = u.detach()
+    sr = scr.clone().requires_grad_(True)
+    w = torch.rand((n, n), generator=g).to(dev)
+
+    def step():
+        sr.grad = None
+        v = AG.propagate_stack(ur, sr, dz, area, wl)
+        (w * (v.real ** 2 + v.imag ** 2)).sum().backward()
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        step()
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / 3
+    res["grad_step"] = {"ms": round(el * 1e3, 3), "ms_per_plane": round(el * 1e3 / K, 3)}
     res["workload"] = ("8 x 4096^2 complex64, 5 shared phase screens, 2 mm spacing, 800 nm, "
-                       "area 2 mm: fused stack vs 5 propagations")
-    del u, scr, out, tmp
+                       "area 2 mm: fused stack vs 5 propagations, fused stack adjoint vs 5 "
+                       "adjoints, and one checkpointed gradient step (loss sum(w |u_5|^2))")
+    del u, scr, out, tmp, sr, w
     return res
 
 

@@ -247,6 +247,30 @@ int lp2d_propagate_stack(int n, int batch, int dtype, const void* field_in, void
                          const float* screens, int nscreens, const lp2d_optics* optics,
                          void* const* planes_out, void* stream);
 
+/* Adjoint of lp2d_propagate_stack (same n, batch, screens and optics): from grad_out
+ * = dL/d(field_out) to grad_in = dL/d(field_in) and, when grad_screens is not NULL,
+ * the batch-summed screen gradients (nscreens float32 (n, n) planes, plane k at
+ * grad_screens + k*n*n), with lp2d_propagate_adjoint's conventions.  planes_in:
+ * nscreens host-side pointers, planes_in[k] = u_k, the input of plane k
+ * (planes_in[0] = field_in; the others as lp2d_propagate_stack's planes_out gives
+ * them); none may alias grad_in.  grad_in is the passes' intermediate plane (its
+ * contents undefined with LP2D_ADJ_NO_GRAD_IN) and may alias grad_out.  work: device
+ * memory of work_bytes >= lp2d_propagate_stack_adjoint_ws_size(n, batch, dtype)
+ * (LP2D_E_SIZE otherwise; NULL allowed when that is 0 or grad_screens is NULL), not
+ * aliasing the other arrays, contents undefined on return.  For complex64 sizes with
+ * a block-tiled intermediate the inverse row pass of plane k, the fold and gradient
+ * of screen k and the forward row pass of plane k-1's adjoint run as one sweep
+ * (2*nscreens + 1 sweeps in all), else one lp2d_propagate_adjoint per plane.  No
+ * counterpart in the reference (it has no gradients). */
+int lp2d_propagate_stack_adjoint(int n, int batch, int dtype, const void* grad_out,
+                                 const void* const* planes_in, const float* screens,
+                                 int nscreens, const lp2d_optics* optics, void* grad_in,
+                                 float* grad_screens, int adj_flags, void* work,
+                                 int64_t work_bytes, void* stream);
+/* Bytes of work lp2d_propagate_stack_adjoint needs for (n, batch, dtype) (0: none;
+ * -1: invalid arguments). */
+int64_t lp2d_propagate_stack_adjoint_ws_size(int n, int batch, int dtype);
+
 /* Adjoint (vector-Jacobian product) of lp2d_propagate, for a real loss with
  * PyTorch's complex-gradient convention (grad = dL/dRe + i dL/dIm):
  *   g1 = ifft2(conj(H) * aperture * fft2(grad_out))

@@ -18,7 +18,9 @@ has no gradients; the forward is the sequence coordinate_filter(exp(i*phase)) ->
 planes: only every s-th plane input is kept in the forward; the backward recomputes
 each segment from its checkpoint before running the adjoints in reverse.  Forward and
 recomputation run as fused stacks (lp2d_propagate_stack: 2K + 1 sweeps for K planes,
-the checkpoints stored by the chained row passes).
+the checkpoints stored by the chained row passes), and the adjoint of a recomputed
+segment as one fused stack adjoint (lp2d_propagate_stack_adjoint, 2m + 1 sweeps for
+m planes).
 """
 from __future__ import annotations
 
@@ -118,7 +120,8 @@ class StackFn(torch.autograd.Function):
         (screens,) = ctx.saved_tensors
         dz, area, wl, stride, K, recompute = ctx.cfg
         g = grad_out.contiguous()
-        gscreens = torch.zeros_like(screens) if ctx.needs_input_grad[1] else None
+        gscreens = (torch.zeros(screens.shape, dtype=screens.dtype, device=screens.device)
+                    if ctx.needs_input_grad[1] else None)
 
         screens_c = screens.contiguous()
 
@@ -139,8 +142,18 @@ class StackFn(torch.autograd.Function):
                     last = P.propagate_stack(ck, screens_c[seg0:seg1 - 1], dz, area, wl,
                                              planes=pl)
                     planes += pl[1:] + [last]
+            if planes is not None:
+                # the segment's adjoint as one fused stack adjoint (2m + 1 sweeps)
+                need_g = seg0 > 0 or ctx.needs_input_grad[0]
+                g, _ = P.propagate_stack_adjoint(
+                    g, planes, screens_c[seg0:seg1], dz, area, wl,
+                    want_screen_grad=gscreens is not None, want_field_grad=need_g,
+                    out=g if g is not grad_out else None,
+                    grad_screens=gscreens[seg0:seg1] if gscreens is not None else None)
+                del planes
+                continue
             for k in reversed(range(seg0, seg1)):
-                uk = planes[k - seg0] if planes is not None else advance(ck, seg0, k)
+                uk = advance(ck, seg0, k)
                 need_g = k > 0 or ctx.needs_input_grad[0]
                 g, gphi = P.propagate_adjoint(g, uk, dz, area, wl, phase=screens[k],
                                               want_phase_grad=gscreens is not None,
@@ -148,7 +161,6 @@ class StackFn(torch.autograd.Function):
                 if gscreens is not None:
                     gscreens[k].copy_(gphi)
                 del uk
-            del planes
         ctx.ckpts = None
         return (g if ctx.needs_input_grad[0] else None, gscreens, None, None, None, None, None)
 

@@ -866,6 +866,113 @@ int op_adjoint(int n, int lg, int batch, const void* gout, const void* u, const 
   return LP2D_OK;
 }
 
+// Field groups of the stack adjoint's chained pass (about 512K threads in flight, no
+// empty group, as grad_groups).
+template <typename T>
+int adj_chain_groups(int lg, int batch) {
+  const Registry<T>* r = reg<T>();
+  if (!r->adj_chain[lg][0]) return 1;
+  const int tpf = (1 << lg) / r->adj_lines[lg];
+  int groups = (2048 * 256 / r->adj_threads[lg]) / tpf;
+  if (groups < 1) groups = 1;
+  if (groups > batch) groups = batch;
+  const int per = (batch + groups - 1) / groups;
+  return (batch + per - 1) / per;
+}
+
+// Float workspace of op_stack_adjoint: the per-group (chained pass) or per-field /
+// per-group (plane-by-plane adjoint) phase-gradient planes of the shared screens.
+template <typename T>
+size_t stack_adjoint_ws(int lg, int batch) {
+  if (batch < 2) return 0;
+  const size_t plane = ((size_t)1 << (2 * lg)) * sizeof(float);
+  const int gc = adj_chain_groups<T>(lg, batch);
+  const size_t chain = gc > 1 ? (size_t)gc * plane : 0;
+  const int gg = grad_groups<T>(lg, batch);
+  const size_t per = use_small<T>(lg) ? (size_t)batch * plane : (gg > 1 ? (size_t)gg * plane : 0);
+  return chain > per ? chain : per;
+}
+
+// Adjoint of op_stack: from grad_out = dL/du_K back to dL/du_0 (grad_in) and the
+// batch-summed screen gradients.  With a block-tiled intermediate it runs
+//   forward rows -> [cols(-z) -> adjoint chain(inverse rows, fold screen k, gradient
+//   of screen k, forward rows)] x (K-1) -> cols(-z) -> last chain (natural store),
+// 2K + 1 sweeps in grad_in; otherwise one adjoint per plane.  uk[k] = u_k, the input
+// of plane k (the chained form reads it only for the screen gradients).
+template <typename T>
+int op_stack_adjoint(int n, int lg, int batch, const void* gout, const void* const* uk,
+                     const float* screens, int K, const lp2d_optics* o, void* gin,
+                     float* gscreens, int aflags, float* ws, hipStream_t s) {
+  const long long nn = (long long)n * n;
+  DeviceTables* t;
+  if (int rc = current_tables(&t, s)) return rc;
+  PassArgs<T> a = base_args<T>(t, n, batch);
+  set_optics(a, n, o);
+  a.z = -o->z;
+  int mid = 0;
+  if (o->z != 0.0) mid |= F_H;
+  if (o->aperture_radius > 0.0) mid |= F_APERTURE;
+  a.flags = mid;
+  const int tl = use_small<T>(lg) ? 0 : tiled_flag<T>(a, lg);
+  const RowAdjChainLaunch<T> mid_fn = reg<T>()->adj_chain[lg][0];
+  const RowAdjChainLaunch<T> last_fn = reg<T>()->adj_chain[lg][1];
+  if (!tl || !mid_fn || !last_fn || o->lens_f != 0.0) {
+    const void* cur = gout;
+    for (int k = K - 1; k >= 0; --k) {
+      const int fl = k > 0 ? 0 : (aflags & LP2D_ADJ_NO_GRAD_IN);
+      if (int rc = op_adjoint<T>(n, lg, batch, cur, uk ? uk[k] : nullptr, screens + k * nn, 0, o,
+                                 gin, gscreens ? gscreens + k * nn : nullptr, fl, nullptr, s,
+                                 ws))
+        return rc;
+      cur = gin;
+    }
+    return LP2D_OK;
+  }
+  const int groups = adj_chain_groups<T>(lg, batch);
+  a.src = (const cx<T>*)gout;
+  a.dst = (cx<T>*)gin;
+  a.flags = tl;
+  if (int rc = run_rows<T>(a, lg, false, s)) return rc;
+  const T sc = (T)(1.0 / ((double)n * n));
+  for (int k = K - 1; k >= 0; --k) {
+    PassArgs<T> c = base_args<T>(t, n, batch);
+    set_optics(c, n, o);
+    c.z = -o->z;
+    c.src = (const cx<T>*)gin;
+    c.dst = (cx<T>*)gin;
+    c.flags = mid | tl;
+    if (int rc = run_cols<T>(c, lg, MODE_FWD | MODE_INV, s)) return rc;
+    PassArgs<T> r = base_args<T>(t, n, batch);
+    r.src = (const cx<T>*)gin;
+    r.dst = (cx<T>*)gin;
+    r.scale = sc;
+    r.phase = screens + k * nn;
+    r.phase_sb = 0;
+    r.u_saved = gscreens ? (const cx<T>*)uk[k] : nullptr;
+    r.u_sb = nn;
+    r.flags = F_SCALE | tl | (k == 0 && (aflags & LP2D_ADJ_NO_GRAD_IN) ? F_NO_STORE : 0);
+    float* gk = gscreens ? gscreens + k * nn : nullptr;
+    float* part = gk && groups > 1 ? ws : gk;
+    std::string nm;
+    if (g_trace_on) {
+      char b[64];
+      snprintf(b, sizeof(b), "rows_adj_chain n=%d %s%s", n, tname<T>(), k ? "" : " last");
+      nm = b;
+    }
+    hipError_t e;
+    {
+      TraceScope ts(s, nm);
+      e = (k ? mid_fn : last_fn)(r, groups, part, s);
+    }
+    if (e != hipSuccess) return hip_fail(e, "adjoint chained row pass launch");
+    if (part && part != gk) {
+      k_sum_planes<<<grid_for(nn), 256, 0, s>>>(part, gk, nn, groups);
+      if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "sum kernel launch");
+    }
+  }
+  return LP2D_OK;
+}
+
 // ----------------------------------------------------------------- uneven batches
 // Fields are grouped by size; each size bucket keeps device tables of its fields'
 // source / destination pointers and optics, and runs through the tuned per-size
@@ -1122,6 +1229,47 @@ int lp2d_propagate_adjoint_noalloc(int n, int batch, int dtype, const void* grad
   return dtype == LP2D_C64
              ? op_adjoint<float>(n, lg, batch, grad_out, field_in, phase, phase_stride_b, optics, grad_in, grad_phase, adj_flags, nullptr, s, sp)
              : op_adjoint<double>(n, lg, batch, grad_out, field_in, phase, phase_stride_b, optics, grad_in, grad_phase, adj_flags, nullptr, s, sp);
+}
+
+int64_t lp2d_propagate_stack_adjoint_ws_size(int n, int batch, int dtype) {
+  int lg;
+  if (check_n(n, batch, dtype, &lg)) return -1;
+  ensure_registry();
+  return (int64_t)(dtype == LP2D_C64 ? stack_adjoint_ws<float>(lg, batch)
+                                     : stack_adjoint_ws<double>(lg, batch));
+}
+
+int lp2d_propagate_stack_adjoint(int n, int batch, int dtype, const void* grad_out,
+                                 const void* const* planes_in, const float* screens,
+                                 int nscreens, const lp2d_optics* optics, void* grad_in,
+                                 float* grad_screens, int adj_flags, void* work,
+                                 int64_t work_bytes, void* stream) {
+  int lg;
+  if (int rc = check_n(n, batch, dtype, &lg)) return rc;
+  if (!grad_out || !grad_in || !optics || !screens) return fail(LP2D_E_ARG, "null pointer");
+  if (nscreens < 1) return fail(LP2D_E_ARG, "nscreens must be >= 1");
+  if (int rc = check_optics(optics)) return rc;
+  if (adj_flags & ~LP2D_ADJ_NO_GRAD_IN) return fail(LP2D_E_ARG, "unknown adj_flags bits");
+  if (!planes_in) return fail(LP2D_E_ARG, "null planes_in");
+  for (int k = 0; k < nscreens; ++k) {
+    if (!planes_in[k]) return fail(LP2D_E_ARG, "planes_in[%d] is null", k);
+    if (planes_in[k] == grad_in) return fail(LP2D_E_ARG, "grad_in must not alias planes_in[%d]", k);
+  }
+  ensure_registry();
+  const size_t need = dtype == LP2D_C64 ? stack_adjoint_ws<float>(lg, batch)
+                                        : stack_adjoint_ws<double>(lg, batch);
+  if (grad_screens && need > 0) {
+    if (!work || work_bytes < (int64_t)need)
+      return fail(LP2D_E_SIZE, "n=%d batch=%d: work needs %zu bytes, got %lld", n, batch, need,
+                  (long long)work_bytes);
+    if (work == grad_in || work == grad_out || work == grad_screens)
+      return fail(LP2D_E_ARG, "work must not alias grad_out, grad_in or grad_screens");
+  }
+  hipStream_t s = (hipStream_t)stream;
+  float* ws = grad_screens && need > 0 ? (float*)work : nullptr;
+  return dtype == LP2D_C64
+             ? op_stack_adjoint<float>(n, lg, batch, grad_out, planes_in, screens, nscreens, optics, grad_in, grad_screens, adj_flags, ws, s)
+             : op_stack_adjoint<double>(n, lg, batch, grad_out, planes_in, screens, nscreens, optics, grad_in, grad_screens, adj_flags, ws, s);
 }
 
 int lp2d_batch_create(const lp2d_field_desc* descs, int count, int dtype, lp2d_batch** out) {

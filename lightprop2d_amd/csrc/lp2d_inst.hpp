@@ -313,6 +313,26 @@ hipError_t launch_rows_chain(const PassArgs<T>& a, hipStream_t s) {
   }
 }
 
+// Adjoint chained pass: grid (N/LINES row tiles) x (field groups); twiddles in LDS
+// (TWL) or from the global tables.
+template <typename T, int N, int E, int LINES, int PS, bool CL, bool TWL, bool LAST>
+__global__ __launch_bounds__((N / E) * LINES) LP2D_OCC
+void k_rows_adj_chain(PassArgs<T> a, int groups, float* partial) {
+  __shared__ cx<T> lds[LINES * RowPitch<N, PS, LINES>::value];
+  __shared__ cx<T> twl[TWL ? TwlLen<N, E, CL>::value : 1];
+  constexpr int TPF = N / LINES;
+  const int tile = blockIdx.x % TPF, g = blockIdx.x / TPF;
+  const int per = (a.batch + groups - 1) / groups;
+  const int b0 = g * per, b1 = cmin(a.batch, b0 + per);
+  if constexpr (TWL) {
+    twl_fill<T, N, E, (N / E) * LINES, 1, CL>(twl, a.tw, threadIdx.x);
+    __syncthreads();
+  }
+  rows_adj_chain_body<T, N, E, LINES, PS, CL, LAST>(
+      a, b0, b1, tile * LINES, partial ? partial + (long long)g * N * N : nullptr, lds,
+      TWL ? twl : nullptr, threadIdx.x);
+}
+
 template <typename T, int N, int E, int LINES, bool CL>
 constexpr int chain_l_lds() {
   return (LINES * RowPitch<N, cmax(1, ilog2c(E)), LINES>::value + TwlLen<N, E, CL>::value) *
@@ -333,6 +353,33 @@ hipError_t launch_rows_chain_l(const PassArgs<T>& a, hipStream_t s) {
     return hipGetLastError();
   } else {
     (void)a;
+    (void)s;
+    return hipErrorInvalidValue;
+  }
+}
+
+template <typename T, int N, int E, int LINES, bool CL, bool TWL>
+constexpr int adj_chain_lds() {
+  return (LINES * RowPitch<N, cmax(1, ilog2c(E)), LINES>::value +
+          (TWL ? TwlLen<N, E, CL>::value : 0)) *
+         (int)sizeof(cx<T>);
+}
+template <typename T, int N, int E, int LINES, int PS, bool CL, bool TWL, bool LAST>
+hipError_t launch_rows_adj_chain(const PassArgs<T>& a, int groups, float* partial, hipStream_t s) {
+  if constexpr (chain_ok<T, N, E, LINES, PS>() &&
+                adj_chain_lds<T, N, E, LINES, CL, TWL>() <= kLdsBudget) {
+    if (!(a.flags & F_TILED) || !a.phase || (partial && !a.u_saved) || a.src_tab || a.dst_tab ||
+        groups < 1)
+      return hipErrorInvalidValue;
+    constexpr int threads = (N / E) * LINES;
+    const long long grid = (long long)(N / LINES) * groups;
+    k_rows_adj_chain<T, N, E, LINES, PS, CL, TWL, LAST><<<(unsigned)grid, threads, 0, s>>>(
+        a, groups, partial);
+    return hipGetLastError();
+  } else {
+    (void)a;
+    (void)groups;
+    (void)partial;
     (void)s;
     return hipErrorInvalidValue;
   }
@@ -580,6 +627,8 @@ void register_n(Registry<T>& r) {
   // 256 x128: 36 / 34 / 33.
   r.nchain[lg] = 0;
   for (int k = 0; k < 4; ++k) r.chain_twl[lg][k] = 0;
+  r.adj_chain[lg][0] = r.adj_chain[lg][1] = nullptr;
+  r.adj_threads[lg] = r.adj_lines[lg] = 0;
   if constexpr (F32 && N >= 256) {
     using S = RowShape<T, N, 32>;
     constexpr int L16 = cmax(tile_rows(N), 256 / (N / 16));
@@ -626,6 +675,21 @@ void register_n(Registry<T>& r) {
       add16();
       add32();
       add32l();
+    }
+    // the stack's adjoint: 16 elements per thread in row tiles of whole blocks (the
+    // group's gradient sums stay in registers beside two transforms: with 32 elements
+    // the chained form spilled 280-400 bytes per lane, and so does a 1024-thread
+    // workgroup held to 128 registers, so 2048 and 8192 keep the plane-by-plane
+    // adjoint), compact last-pass twiddles in LDS where two workgroups per CU keep
+    // room for them, the global tables otherwise
+    constexpr int LA = L16;
+    constexpr bool TWA = adj_chain_lds<T, N, 16, LA, true, true>() <= 80 * 1024;
+    if constexpr (ok16 && (N / 16) * LA <= 512 &&
+                  adj_chain_lds<T, N, 16, LA, true, TWA>() <= kLdsBudget) {
+      r.adj_chain[lg][0] = &launch_rows_adj_chain<T, N, 16, LA, PS16, true, TWA, false>;
+      r.adj_chain[lg][1] = &launch_rows_adj_chain<T, N, 16, LA, PS16, true, TWA, true>;
+      r.adj_threads[lg] = (N / 16) * LA;
+      r.adj_lines[lg] = LA;
     }
   }
   using SS = SmallShape<T, N, EMAX>;

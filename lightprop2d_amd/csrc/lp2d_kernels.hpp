@@ -672,6 +672,85 @@ __device__ __forceinline__ void rows_chain_body(const PassArgs<T>& a, int b, int
   for (int m = 0; m < E; ++m) gst<NT_ROW_ST>(dst + (lo_t + TS * (unsigned)m), v[m]);
 }
 
+// Adjoint of a multi-plane stack, one sweep per plane.  For the fields [b0, b1) of a
+// group and one row tile (whole row blocks of the block-tiled plane, in place): load
+// the tile, inverse row FFT, 1/n^2, fold g = conj(exp(i*phase[y, x])) * g1 (the screen
+// shared by the batch), add Im(conj(u) * g) to the group's phase-gradient sums (u:
+// the plane's saved input, natural order), then either (LAST) store g in natural
+// order -- the gradient of the stack's input, unless F_NO_STORE -- or run the forward
+// row FFT of g and store the tile in place: the first sweep of the previous plane's
+// adjoint.  The group's sums go to `partial` (n x n) once; with partial null no
+// gradient is formed (u is not read).  Twiddles from the LDS tables twl, or from the
+// global tables when twl is null.
+template <typename T, int N, int E, int LINES, int PS, bool CL, bool LAST>
+__device__ __forceinline__ void rows_adj_chain_body(const PassArgs<T>& a, int b0, int b1, int y0,
+                                                    float* __restrict__ partial, cx<T>* lds,
+                                                    const cx<T>* twl, int tid) {
+  constexpr int TL = N / E;
+  constexpr int L = RowPitch<N, PS, LINES>::value;
+  constexpr int TR = tile_rows(N);
+  using RM = TiledRowMap<N, TL, LINES>;
+  static_assert(RM::ok && N % LINES == 0, "whole tiles of whole row blocks");
+  int l, t;
+  RM::map(tid, l, t);
+  const int y = y0 + l;
+  const long long ubase = (long long)y0 * N;
+  const unsigned lo_t = tiled_off<N>(l, t);
+  constexpr unsigned TS = TR * TL;
+  const bool store = (a.flags & F_NO_STORE) == 0;
+  LineAcc<T, PS> acc{lds + l * L};
+  float gsum[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) gsum[m] = 0.f;
+  for (int b = b0; b < b1; ++b) {
+    // opaque per field: otherwise the twiddle reads (and the screen factors), the same
+    // for every field, are hoisted out of the loop and held across it
+    int tt = t;
+    asm volatile("" : "+v"(tt));
+    cx<T> v[E];
+    const LP2D_GLB cx<T>* __restrict__ src = field_src(a, b) + ubase;
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = gld<NT_ROW_LD>(src + (lo_t + TS * (unsigned)m));
+    if (twl)
+      line_fft_l<T, N, E, true, CL>(v, tt, acc, twl);
+    else
+      line_fft<T, N, E, true>(v, tt, acc, a.tw_inv + (N - 2));
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = cscale(v[m], a.scale);
+    phase_mul_all<T, E, true>(a, v, b, y, tt, TL, N);
+    if (partial) {
+      const LP2D_GLB cx<T>* __restrict__ us =
+          glb(uniform_ptr(a.u_saved + (long long)b * a.u_sb)) + (long long)y * N;
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const cx<T> u = gld<NT_GRAD>(us + (t + TL * m));
+        gsum[m] += (float)(u.re * v[m].im - u.im * v[m].re);
+      }
+    }
+    if constexpr (LAST) {
+      if (store) {
+        LP2D_GLB cx<T>* __restrict__ dst = field_dst(a, b) + (long long)y * N;
+#pragma unroll
+        for (int m = 0; m < E; ++m) gst<NT_ROWI_ST>(dst + (t + TL * m), v[m]);
+      }
+    } else {
+      int t2 = tt;
+      asm volatile("" : "+v"(t2));
+      if (twl)
+        line_fft_l<T, N, E, false, CL>(v, t2, acc, twl);
+      else
+        line_fft<T, N, E, false>(v, t2, acc, a.tw + (N - 2));
+      LP2D_GLB cx<T>* __restrict__ dst = field_dst(a, b) + ubase;
+#pragma unroll
+      for (int m = 0; m < E; ++m) gst<NT_ROW_ST>(dst + (lo_t + TS * (unsigned)m), v[m]);
+    }
+  }
+  if (partial) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) partial[(long long)y * N + t + TL * m] = gsum[m];
+  }
+}
+
 // Row tiles of LINES rows with software pipelining: the tile of step k+1 is loaded
 // into registers while tile k is transformed, as in the pipelined column pass.  A
 // workgroup takes tiles u0, u0 + ustep, u0 + 2*ustep, ... of the whole batch, so the

@@ -20,6 +20,9 @@ using RowGradLaunch = hipError_t (*)(const PassArgs<T>&, int groups, float* part
                                      hipStream_t);
 template <typename T>
 using RowChainLaunch = hipError_t (*)(const PassArgs<T>&, hipStream_t);
+template <typename T>
+using RowAdjChainLaunch = hipError_t (*)(const PassArgs<T>&, int groups, float* partial,
+                                         hipStream_t);
 
 template <typename T>
 struct RowVar {
@@ -59,6 +62,10 @@ struct Registry {
   int chain_E[kLogMax + 1][4], chain_lines[kLogMax + 1][4];
   int chain_twl[kLogMax + 1][4];  // 1: twiddles staged in LDS
   int nchain[kLogMax + 1];
+  // adjoint of a stack: [0] chained (inverse rows, fold, gradient, forward rows),
+  // [1] last plane (inverse rows, fold, gradient, natural store); threads per group
+  RowAdjChainLaunch<T> adj_chain[kLogMax + 1][2];
+  int adj_threads[kLogMax + 1], adj_lines[kLogMax + 1];
 };
 
 // Workgroups of a column-pass kernel resident at once on the current device.

@@ -16,7 +16,8 @@ from . import _lib as L
 from ._lib import Lp2dError
 
 __all__ = ["fft2", "ifft2", "kz_grid", "spatial_apply", "spectral_apply", "propagate",
-           "propagate_adjoint", "propagate_adjoint_noalloc", "propagate_stack", "intensity", "sum_planes", "UnevenBatch", "optics",
+           "propagate_adjoint", "propagate_adjoint_noalloc", "propagate_stack",
+           "propagate_stack_adjoint", "intensity", "sum_planes", "UnevenBatch", "optics",
            "expand", "coarse", "mode_matrix", "modes_project", "modes_construct",
            "Lp2dError"]
 
@@ -346,6 +347,71 @@ def propagate_adjoint(grad_out: torch.Tensor, field_in: Optional[torch.Tensor], 
     if gph is not None and psb == 0:
         gph = gph[0]
     return (out if want_field_grad else None), gph
+
+
+def propagate_stack_adjoint(grad_out: torch.Tensor, planes: Sequence[torch.Tensor],
+                            screens: torch.Tensor, z: float, area_size: float, wl: float,
+                            aperture_radius: float = 0.0, lens_f: float = 0.0,
+                            evanescent: str = "reference", want_screen_grad: bool = True,
+                            want_field_grad: bool = True, out: Optional[torch.Tensor] = None,
+                            grad_screens: Optional[torch.Tensor] = None):
+    """Adjoint of `propagate_stack`: returns (grad_field, grad_screens).
+
+    planes[k] = u_k, the input of plane k (planes[0] = the stack's input field), as
+    `propagate_stack(..., planes=...)` stores them.  grad_screens: (K, n, n) float32,
+    the batch-summed screen gradients (None when want_screen_grad is False).
+    grad_field is None when want_field_grad is False.  out may be grad_out (in place).
+    Complex64 sizes with a block-tiled intermediate run 2K + 1 sweeps: the inverse row
+    pass of plane k, the fold and gradient of screen k and the forward row pass of
+    plane k-1's adjoint are one sweep."""
+    lib = _lib()
+    n, batch, dt = _field_geom(grad_out, "grad_out")
+    if not isinstance(screens, torch.Tensor) or screens.dtype != torch.float32 or \
+            screens.dim() != 3 or screens.shape[1:] != (n, n) or \
+            screens.device != grad_out.device or not screens.is_contiguous():
+        raise Lp2dError(f"screens must be a contiguous float32 (K, {n}, {n}) tensor on the "
+                        "field's device")
+    K = screens.shape[0]
+    if K < 1:
+        raise Lp2dError("at least one screen")
+    planes = list(planes)
+    if len(planes) != K:
+        raise Lp2dError(f"planes must hold the {K} plane inputs")
+    if out is None:
+        out = torch.empty_like(grad_out)
+    else:
+        _check_like(out, grad_out, "out")
+    parr = (C.c_void_p * K)()
+    for k, pl in enumerate(planes):
+        _check_like(pl, grad_out, f"planes[{k}]")
+        if pl.data_ptr() == out.data_ptr():
+            raise Lp2dError(f"out must not alias planes[{k}]")
+        parr[k] = pl.data_ptr()
+    gs = None
+    if want_screen_grad:
+        if grad_screens is None:
+            gs = torch.empty((K, n, n), dtype=torch.float32, device=grad_out.device)
+        else:
+            if grad_screens.dtype != torch.float32 or grad_screens.shape != (K, n, n) or \
+                    grad_screens.device != grad_out.device or not grad_screens.is_contiguous():
+                raise Lp2dError(f"grad_screens must be a contiguous float32 ({K}, {n}, {n}) "
+                                "tensor on the field's device")
+            gs = grad_screens
+    need = int(lib.lp2d_propagate_stack_adjoint_ws_size(n, batch, dt))
+    if need < 0:
+        raise Lp2dError(f"invalid stack adjoint geometry n={n} batch={batch}")
+    work = None
+    if gs is not None and need > 0:
+        work = torch.empty((need + 3) // 4, dtype=torch.float32, device=grad_out.device)
+    o = optics(area_size, wl, z=z, aperture_radius=aperture_radius, lens_f=lens_f,
+               evanescent=evanescent)
+    flags = 0 if want_field_grad else L.LP2D_ADJ_NO_GRAD_IN
+    with torch.cuda.device(grad_out.device):
+        L.check(lib.lp2d_propagate_stack_adjoint(
+            n, batch, dt, _ptr(grad_out), parr, _ptr(screens), K, C.byref(o), _ptr(out),
+            _ptr(gs), flags, _ptr(work), need if work is not None else 0, _stream(grad_out)),
+            "propagate_stack_adjoint")
+    return (out if want_field_grad else None), gs
 
 
 def propagate_adjoint_noalloc(grad_out: torch.Tensor, field_in: Optional[torch.Tensor],

@@ -258,3 +258,30 @@ def test_noalloc_adjoint_rejects_sizes_beyond_its_scratch(lib):
                                               None) == _lib.LP2D_E_ARG
     assert lib.lp2d_propagate_adjoint_noalloc(64, 1, 0, p, p, None, 0, C.byref(o),
                                               C.c_void_p(32), p, 0, None) == _lib.LP2D_E_ARG
+
+
+def test_stack_adjoint_checks_arguments_and_workspace(lib):
+    """lp2d_propagate_stack_adjoint validates its plane list and workspace before any
+    device work; the workspace query is host-only: per-group gradient planes of a
+    shared screen over 4 fields at 512^2, none for one field, -1 for a size that is
+    not a power of two."""
+    from lightprop2d_amd import _lib
+    need = lib.lp2d_propagate_stack_adjoint_ws_size(512, 4, _lib.LP2D_C64)
+    assert need > 0 and need % (512 * 512 * 4) == 0
+    assert lib.lp2d_propagate_stack_adjoint_ws_size(512, 1, _lib.LP2D_C64) == 0
+    assert lib.lp2d_propagate_stack_adjoint_ws_size(500, 4, _lib.LP2D_C64) == -1
+    o = _lib.lp2d_optics(1.0, 1e-4, 0.1, 0.0, 0.0, 0, 0)
+    g, gin, scr, gs, work = (C.c_void_p(a) for a in (16, 32, 48, 64, 80))
+    planes = (C.c_void_p * 2)(96, 112)
+
+    def call(pl, k=2, out=gin, wk=work, wb=need):
+        return lib.lp2d_propagate_stack_adjoint(512, 4, _lib.LP2D_C64, g, pl, scr, k,
+                                                C.byref(o), out, gs, 0, wk, wb, None)
+
+    assert call(planes, k=0) == _lib.LP2D_E_ARG
+    assert call(None) == _lib.LP2D_E_ARG
+    assert call((C.c_void_p * 2)(96, None)) == _lib.LP2D_E_ARG
+    assert call(planes, out=C.c_void_p(112)) == _lib.LP2D_E_ARG      # grad_in = u_1
+    assert call(planes, wb=need - 4) == _lib.LP2D_E_SIZE
+    assert call(planes, wk=None) == _lib.LP2D_E_SIZE
+    assert call(planes, wk=gin) == _lib.LP2D_E_ARG                   # work = grad_in

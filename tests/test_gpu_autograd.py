@@ -365,3 +365,105 @@ def test_stack_grad_c64_fused_vs_oracle(stride, mode):
         gs += np.stack(gl)
         assert rel_l2(u_t.grad[b].cpu().numpy(), gu) < 2e-5
     assert rel_l2(s_t.grad.cpu().numpy(), gs) < 2e-5
+
+
+@pytest.mark.parametrize("n,dt,B,K", [(256, torch.complex64, 3, 3), (512, torch.complex64, 3, 4),
+                                      (1024, torch.complex64, 9, 3),
+                                      (4096, torch.complex64, 1, 2),
+                                      (2048, torch.complex64, 2, 2), (128, torch.complex64, 3, 3),
+                                      (256, torch.complex128, 2, 3)])
+def test_fused_stack_adjoint_vs_oracle(n, dt, B, K):
+    """lp2d_propagate_stack_adjoint with a pupil, given the plane inputs u_0 .. u_{K-1}
+    and grad_out: the gradient of the stack's input (field by field) and the batch-summed
+    screen gradients against the oracle's reverse sweep.  256, 512, 1024 and 4096
+    complex64 run the chained adjoint (1024 with 9 fields: uneven field groups whose
+    gradient planes are summed after each plane); 2048, 128^2 and complex128 the
+    plane-by-plane adjoint.  Both forms agree with a plane-by-plane composition of
+    lp2d_propagate_adjoint; in place (out = grad_out) the results are unchanged, and
+    without screen gradients the field gradient is unchanged (the chained form) or
+    equal to rounding (plane by plane: with no shared-screen sum the intermediate may
+    take another layout)."""
+    from lightprop2d_amd import propagation as P
+    area, wl, dz = 300e-4, 800e-7, 0.02
+    ap = 0.35 / (area / n)
+    rs = np.random.RandomState(7 * n + B)
+    c = np.complex64 if dt == torch.complex64 else np.complex128
+    u0 = (rs.standard_normal((B, n, n)) + 1j * rs.standard_normal((B, n, n))).astype(c)
+    scr = rs.uniform(0, 2 * np.pi, (K, n, n)).astype(np.float32)
+    gout = (rs.standard_normal((B, n, n)) + 1j * rs.standard_normal((B, n, n))).astype(c)
+    planes = [u0]
+    for k in range(K - 1):
+        planes.append(oracle.propagate_field(planes[-1], area, wl, dz,
+                                             phase_mask=scr[k].astype(np.float64),
+                                             aperture_radius=ap).astype(c))
+    g = gout.astype(np.complex128)
+    gs_ref = [None] * K
+    for k in range(K - 1, -1, -1):
+        g, gk = oracle.propagate_field_adjoint(g, planes[k], area, wl, dz,
+                                               phase_mask=scr[k].astype(np.float64),
+                                               aperture_radius=ap)
+        gs_ref[k] = gk.sum(axis=0)
+    tp = [torch.from_numpy(p).to(DEV) for p in planes]
+    ts = torch.from_numpy(scr).to(DEV)
+    tg = torch.from_numpy(gout).to(DEV)
+    print(f"stack adjoint n={n} {dt} B={B} K={K}", flush=True)
+    gu, gs = P.propagate_stack_adjoint(tg, tp, ts, dz, area, wl, aperture_radius=ap)
+    tol = 1e-5 if dt == torch.complex64 else 1e-10
+    for b in range(B):
+        assert rel_l2(gu[b].cpu().numpy(), g[b]) < tol, b
+    for k in range(K):
+        # float32 gradient planes summed over the batch
+        assert rel_l2(gs[k].cpu().numpy(), gs_ref[k]) < (2e-5 if dt == torch.complex64
+                                                          else 1e-6), k
+    # plane by plane with lp2d_propagate_adjoint
+    gseq = tg
+    for k in range(K - 1, -1, -1):
+        gseq, gk = P.propagate_adjoint(gseq, tp[k], dz, area, wl, phase=ts[k],
+                                       aperture_radius=ap)
+        assert rel_l2(gs[k].cpu().numpy(), gk.cpu().numpy()) < 1e-5, k
+    assert rel_l2(gu.cpu().numpy(), gseq.cpu().numpy()) < tol / 10
+    # no screen gradients; in place
+    gu2, none = P.propagate_stack_adjoint(tg, tp, ts, dz, area, wl, aperture_radius=ap,
+                                          want_screen_grad=False)
+    assert none is None
+    if n in (256, 512, 1024, 4096) and dt == torch.complex64:
+        assert torch.equal(gu2, gu)
+    else:
+        assert rel_l2(gu2.cpu().numpy(), gu.cpu().numpy()) < tol / 10
+    ip = tg.clone()
+    gu3, gs3 = P.propagate_stack_adjoint(ip, tp, ts, dz, area, wl, aperture_radius=ap, out=ip)
+    assert gu3.data_ptr() == ip.data_ptr() and torch.equal(gu3, gu) and torch.equal(gs3, gs)
+
+
+def test_stack_adjoint_screen_grad_only_and_errors():
+    """LP2D_ADJ_NO_GRAD_IN (screen gradients only) gives the same screen gradients;
+    grad_in aliasing a plane input, too few planes and a short workspace are refused
+    before any launch."""
+    from lightprop2d_amd import propagation as P, _lib as L
+    import ctypes as C
+    n, B, K, area, wl, dz = 512, 4, 3, 300e-4, 800e-7, 0.02
+    rs = np.random.RandomState(11)
+    tp = [torch.from_numpy((rs.standard_normal((B, n, n)) + 1j * rs.standard_normal((B, n, n)))
+                           .astype(np.complex64)).to(DEV) for _ in range(K)]
+    ts = torch.from_numpy(rs.uniform(0, 6, (K, n, n)).astype(np.float32)).to(DEV)
+    tg = torch.from_numpy((rs.standard_normal((B, n, n)) + 0j).astype(np.complex64)).to(DEV)
+    gu, gs = P.propagate_stack_adjoint(tg, tp, ts, dz, area, wl)
+    none, gs2 = P.propagate_stack_adjoint(tg, tp, ts, dz, area, wl, want_field_grad=False)
+    assert none is None and torch.equal(gs2, gs)
+    with pytest.raises(P.Lp2dError):
+        P.propagate_stack_adjoint(tg, tp, ts, dz, area, wl, out=tp[1])
+    with pytest.raises(P.Lp2dError):
+        P.propagate_stack_adjoint(tg, tp[:2], ts, dz, area, wl)
+    lib = L.lib()
+    need = lib.lp2d_propagate_stack_adjoint_ws_size(n, B, L.LP2D_C64)
+    assert need > 0   # several field groups or a shared-screen sum over 4 fields
+    parr = (C.c_void_p * K)(*[p.data_ptr() for p in tp])
+    o = P.optics(area, wl, z=dz)
+    work = torch.empty(need // 4, dtype=torch.float32, device=DEV)
+    out = torch.empty_like(tg)
+    rc = lib.lp2d_propagate_stack_adjoint(n, B, L.LP2D_C64, C.c_void_p(tg.data_ptr()), parr,
+                                          C.c_void_p(ts.data_ptr()), K, C.byref(o),
+                                          C.c_void_p(out.data_ptr()),
+                                          C.c_void_p(gs.data_ptr()), 0,
+                                          C.c_void_p(work.data_ptr()), need - 4, None)
+    assert rc == L.LP2D_E_SIZE
