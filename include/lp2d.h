@@ -14,7 +14,8 @@
  *  - n must be a power of two, 2 <= n <= lp2d_max_n().
  *  - `stream` is a hipStream_t passed as void* (NULL = default stream).  Calls are
  *    stream-ordered and asynchronous.  Device memory is allocated only by lp2d_init
- *    (twiddle tables, once per device) and lp2d_batch_create (work tables).
+ *    (twiddle tables, once per device; lp2d_plan_create runs it) and
+ *    lp2d_batch_create (work tables); workspaces are the caller's.
  *  - Every call returns LP2D_OK (0) or a negative status; lp2d_last_error() gives
  *    the message for the calling thread.  No C++ exception crosses the ABI.
  *  - Forward FFT is unnormalised, inverse scales by 1/n^2 (numpy convention,
@@ -322,6 +323,58 @@ int lp2d_propagate_adjoint_noalloc(int n, int batch, int dtype, const void* grad
 /* Sum `batch` float planes of `count` elements: out[i] = sum_b in[b*count + i].
  * Used to reduce per-field phase gradients of a shared mask. */
 int lp2d_sum_planes(int64_t count, int batch, const float* in, float* out, void* stream);
+
+/* ---------------------------------------------------------------- plans */
+/* A plan fixes one field geometry (n, batch, dtype) on the device current at
+ * lp2d_plan_create, initialises that device (the allocation point: the twiddle
+ * tables), and keeps a copy of the kernel schedule then selected for n
+ * (lp2d_set_variant / lp2d_set_chunk, or the autotuner's table): later selections
+ * change nothing a plan runs, so plans used from several threads each run what they
+ * were made with.  A plan call must run with the plan's device current (LP2D_E_ARG
+ * otherwise), takes the same arguments as the plan-less entry point of the same name
+ * minus (n, batch, dtype), and allocates nothing; the adjoint forms take their
+ * workspace from the caller (lp2d_plan_workspace_size).  No counterpart in the
+ * reference, whose backend is selected per Beam2D (beam2d.py:105-109). */
+typedef struct lp2d_plan lp2d_plan;
+#define LP2D_OP_FFT2 0
+#define LP2D_OP_PROPAGATE 1
+#define LP2D_OP_ADJOINT 2
+#define LP2D_OP_STACK 3
+#define LP2D_OP_STACK_ADJOINT 4
+int lp2d_plan_create(int n, int batch, int dtype, lp2d_plan** out);
+int lp2d_plan_destroy(lp2d_plan* plan);
+/* Bytes of `work` operation `op` (LP2D_OP_*) needs on this plan (0: none; -1: bad
+ * plan or op). */
+int64_t lp2d_plan_workspace_size(const lp2d_plan* plan, int op);
+/* lp2d_fft2 (beam2d.py:207-229). */
+int lp2d_plan_fft2(const lp2d_plan* plan, const void* in, void* out, int inverse, void* stream);
+/* lp2d_propagate (beam2d.py:339-342, 481-490, 370-373). */
+int lp2d_plan_propagate(const lp2d_plan* plan, const void* field_in, void* field_out,
+                        const float* phase, int64_t phase_stride_b, const lp2d_optics* optics,
+                        void* spectrum_out, void* stream);
+/* lp2d_propagate_adjoint with the caller's workspace: `work` holds at least
+ * lp2d_plan_workspace_size(plan, LP2D_OP_ADJOINT) bytes (LP2D_E_SIZE otherwise; NULL
+ * when that is 0), must not alias the other arrays, and is undefined on return.  It
+ * carries the column pass's output (complex64 three-sweep sizes) and, with a shared
+ * screen (phase_stride_b = 0) over a batch, the per-group gradient planes, so that
+ * grad_phase is then ONE n x n plane receiving sum_b gphi[b] (per-field phase:
+ * (batch, n, n) as in lp2d_propagate_adjoint). */
+int lp2d_plan_propagate_adjoint(const lp2d_plan* plan, const void* grad_out,
+                                const void* field_in, const float* phase,
+                                int64_t phase_stride_b, const lp2d_optics* optics,
+                                void* grad_in, float* grad_phase, int adj_flags, void* work,
+                                int64_t work_bytes, void* stream);
+/* lp2d_propagate_stack (beam2d.py:339-342, 481-490 per plane). */
+int lp2d_plan_propagate_stack(const lp2d_plan* plan, const void* field_in, void* field_out,
+                              const float* screens, int nscreens, const lp2d_optics* optics,
+                              void* const* planes_out, void* stream);
+/* lp2d_propagate_stack_adjoint; work of lp2d_plan_workspace_size(plan,
+ * LP2D_OP_STACK_ADJOINT) bytes. */
+int lp2d_plan_propagate_stack_adjoint(const lp2d_plan* plan, const void* grad_out,
+                                      const void* const* planes_in, const float* screens,
+                                      int nscreens, const lp2d_optics* optics, void* grad_in,
+                                      float* grad_screens, int adj_flags, void* work,
+                                      int64_t work_bytes, void* stream);
 
 /* ---------------------------------------------------------------- uneven batches */
 /* One independent field of an uneven batch: its own size, distance and optics. */

@@ -32,6 +32,9 @@ EVANESCENT = {"reference": 0, "pass": 0, "zero": 1, "decay": 2}
 
 LP2D_ADJ_NO_GRAD_IN = 1
 
+# operations of lp2d_plan_workspace_size
+LP2D_OP_FFT2, LP2D_OP_PROPAGATE, LP2D_OP_ADJOINT, LP2D_OP_STACK, LP2D_OP_STACK_ADJOINT = range(5)
+
 # every symbol include/lp2d.h declares
 EXPORTED = [
     "lp2d_init", "lp2d_last_error", "lp2d_version", "lp2d_max_n", "lp2d_kz_grid",
@@ -39,6 +42,9 @@ EXPORTED = [
     "lp2d_propagate", "lp2d_propagate_adjoint", "lp2d_propagate_adjoint_ws",
     "lp2d_propagate_adjoint_noalloc", "lp2d_propagate_stack",
     "lp2d_propagate_stack_adjoint", "lp2d_propagate_stack_adjoint_ws_size",
+    "lp2d_plan_create", "lp2d_plan_destroy", "lp2d_plan_workspace_size", "lp2d_plan_fft2",
+    "lp2d_plan_propagate", "lp2d_plan_propagate_adjoint", "lp2d_plan_propagate_stack",
+    "lp2d_plan_propagate_stack_adjoint",
     "lp2d_sum_planes", "lp2d_batch_create",
     "lp2d_batch_run", "lp2d_batch_destroy", "lp2d_variant_count",
     "lp2d_set_variant", "lp2d_get_variant", "lp2d_variant_name", "lp2d_trace_enable",
@@ -129,6 +135,18 @@ _SIGS = {
     "lp2d_propagate_stack_adjoint": (_I, [_I, _I, _I, _P, C.POINTER(_P), _P, _I,
                                           C.POINTER(lp2d_optics), _P, _P, _I, _P, _I64, _P]),
     "lp2d_propagate_stack_adjoint_ws_size": (_I64, [_I, _I, _I]),
+    "lp2d_plan_create": (_I, [_I, _I, _I, C.POINTER(_P)]),
+    "lp2d_plan_destroy": (_I, [_P]),
+    "lp2d_plan_workspace_size": (_I64, [_P, _I]),
+    "lp2d_plan_fft2": (_I, [_P, _P, _P, _I, _P]),
+    "lp2d_plan_propagate": (_I, [_P, _P, _P, _P, _I64, C.POINTER(lp2d_optics), _P, _P]),
+    "lp2d_plan_propagate_adjoint": (_I, [_P, _P, _P, _P, _I64, C.POINTER(lp2d_optics), _P, _P,
+                                         _I, _P, _I64, _P]),
+    "lp2d_plan_propagate_stack": (_I, [_P, _P, _P, _P, _I, C.POINTER(lp2d_optics),
+                                       C.POINTER(_P), _P]),
+    "lp2d_plan_propagate_stack_adjoint": (_I, [_P, _P, C.POINTER(_P), _P, _I,
+                                               C.POINTER(lp2d_optics), _P, _P, _I, _P, _I64,
+                                               _P]),
     "lp2d_sum_planes": (_I, [_I64, _I, _P, _P, _P]),
     "lp2d_batch_create": (_I, [C.POINTER(lp2d_field_desc), _I, _I, C.POINTER(_P)]),
     "lp2d_batch_run": (_I, [_P, _P]),

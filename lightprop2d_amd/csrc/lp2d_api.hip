@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -71,6 +72,30 @@ Registry<float>* g_rf = nullptr;
 Registry<double>* g_rd = nullptr;
 int g_sel[2][3][kLogMax + 1];  // [dtype][kind: rows, cols, chain][log2 n] -> variant
 int g_chunk[2][kLogMax + 1];   // fields per three-sweep chunk (0 = whole batch)
+
+// A plan's copy of the selection for its (dtype, log2 n), taken at lp2d_plan_create;
+// during a plan call it is the calling thread's selection for that size, so later
+// lp2d_set_variant / lp2d_set_chunk calls do not change what a plan runs.
+struct PlanSel {
+  int dt = -1, lg = -1;
+  int sel[3] = {0, 0, 0};
+  int chunk = 0;
+};
+thread_local const PlanSel* t_plan_sel = nullptr;
+
+int sel_of(int dt, int kind, int lg) {
+  const PlanSel* p = t_plan_sel;
+  return (p && p->dt == dt && p->lg == lg) ? p->sel[kind] : g_sel[dt][kind][lg];
+}
+int chunk_of(int dt, int lg) {
+  const PlanSel* p = t_plan_sel;
+  return (p && p->dt == dt && p->lg == lg) ? p->chunk : g_chunk[dt][lg];
+}
+struct PlanScope {
+  const PlanSel* prev;
+  explicit PlanScope(const PlanSel* s) : prev(t_plan_sel) { t_plan_sel = s; }
+  ~PlanScope() { t_plan_sel = prev; }
+};
 
 void ensure_registry() {
   static std::once_flag once;
@@ -285,7 +310,7 @@ const char* tname() { return sizeof(T) == 4 ? "c64" : "c128"; }
 
 template <typename T>
 int run_rows(const PassArgs<T>& a, int lg, bool inv, hipStream_t s) {
-  const RowVar<T>& v = reg<T>()->rows[lg][g_sel[dt<T>()][0][lg]];
+  const RowVar<T>& v = reg<T>()->rows[lg][sel_of(dt<T>(), 0, lg)];
   std::string nm;
   if (g_trace_on) {
     char b[64];
@@ -298,7 +323,7 @@ int run_rows(const PassArgs<T>& a, int lg, bool inv, hipStream_t s) {
 }
 template <typename T>
 int run_cols(const PassArgs<T>& a, int lg, int mode, hipStream_t s) {
-  const ColVar<T>& v = reg<T>()->cols[lg][g_sel[dt<T>()][1][lg]];
+  const ColVar<T>& v = reg<T>()->cols[lg][sel_of(dt<T>(), 1, lg)];
   std::string nm;
   if (g_trace_on) {
     char b[64];
@@ -343,8 +368,8 @@ template <typename T>
 int tiled_flag(const PassArgs<T>& cols_a, int lg, bool grad = false, bool grad_store = true) {
   if (sizeof(T) != 4) return 0;
   const int n = 1 << lg;
-  const ColVar<T>& cv = reg<T>()->cols[lg][g_sel[dt<T>()][1][lg]];
-  const RowVar<T>& rv = reg<T>()->rows[lg][g_sel[dt<T>()][0][lg]];
+  const ColVar<T>& cv = reg<T>()->cols[lg][sel_of(dt<T>(), 1, lg)];
+  const RowVar<T>& rv = reg<T>()->rows[lg][sel_of(dt<T>(), 0, lg)];
   const int tr = tile_rows(n), tc = tile_cols(n);
   if (!(cv.kind & 2) || cv.W != tc || (n / cv.E) % tr != 0) return 0;
   if (!pipe_eligible(cols_a, MODE_FWD | MODE_INV)) return 0;
@@ -625,7 +650,7 @@ int op_propagate_chunk(int n, int lg, int batch, const void* in, void* out, cons
 template <typename T>
 int op_propagate(int n, int lg, int batch, const void* in, void* out, const float* phase,
                  int64_t phase_sb, const lp2d_optics* o, void* spec, hipStream_t s) {
-  const int chunk = g_chunk[dt<T>()][lg];
+  const int chunk = chunk_of(dt<T>(), lg);
   if (chunk <= 0 || chunk >= batch || use_small<T>(lg))
     return op_propagate_chunk<T>(n, lg, batch, in, out, phase, phase_sb, o, spec, s);
   const long long fs = (long long)n * n;
@@ -720,7 +745,7 @@ int op_stack(int n, int lg, int batch, const void* in, void* out, const float* s
   if (o->aperture_radius > 0.0) mid |= F_APERTURE;
   a.flags = mid;
   const int tl = use_small<T>(lg) ? 0 : tiled_flag<T>(a, lg);
-  const int cv = g_sel[dt<T>()][2][lg];
+  const int cv = sel_of(dt<T>(), 2, lg);
   const RowChainLaunch<T> chain = reg<T>()->nchain[lg] > cv ? reg<T>()->chain[lg][cv] : nullptr;
   if (!tl || !chain || K < 2 || o->lens_f != 0.0) {
     const void* cur = in;
@@ -1014,6 +1039,44 @@ int run_bucket(const Bucket& bk, hipStream_t s) {
 
 }  // namespace
 
+// A plan: one field geometry on one device, with the selection it was created under.
+struct lp2d_plan {
+  int n = 0, lg = 0, batch = 0, dtype = 0, device = 0;
+  PlanSel sel;
+};
+
+namespace {
+
+// The plan's device must be current (its tables and selection belong to it).
+int plan_enter(const lp2d_plan* p) {
+  if (!p) return fail(LP2D_E_ARG, "null plan");
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev != p->device)
+    return fail(LP2D_E_ARG, "plan made on device %d used with device %d current", p->device, dev);
+  return LP2D_OK;
+}
+
+// Bytes of the adjoint workspace: the column pass's output plane (complex64 sizes run
+// in three sweeps), then the per-group (or per-field) gradient planes of a shared
+// screen, each part 256-byte aligned.
+template <typename T>
+void adjoint_ws_parts(int n, int lg, int batch, size_t* plane_bytes, size_t* grad_bytes) {
+  const size_t nn = (size_t)n * n;
+  const bool small = use_small<T>(lg);
+  *plane_bytes = (!small && sizeof(T) == 4) ? (size_t)batch * nn * sizeof(cx<T>) : 0;
+  size_t g = 0;
+  if (batch > 1) {
+    const int gg = grad_groups<T>(lg, batch);
+    g = (small ? (size_t)batch : (gg > 1 ? (size_t)gg : 0)) * nn * sizeof(float);
+  }
+  *grad_bytes = g;
+}
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+
 struct lp2d_batch {
   int device = 0;
   int dtype = 0;
@@ -1270,6 +1333,129 @@ int lp2d_propagate_stack_adjoint(int n, int batch, int dtype, const void* grad_o
   return dtype == LP2D_C64
              ? op_stack_adjoint<float>(n, lg, batch, grad_out, planes_in, screens, nscreens, optics, grad_in, grad_screens, adj_flags, ws, s)
              : op_stack_adjoint<double>(n, lg, batch, grad_out, planes_in, screens, nscreens, optics, grad_in, grad_screens, adj_flags, ws, s);
+}
+
+int lp2d_plan_create(int n, int batch, int dtype, lp2d_plan** out) {
+  if (!out) return fail(LP2D_E_ARG, "null output handle");
+  *out = nullptr;
+  int lg;
+  if (int rc = check_n(n, batch, dtype, &lg)) return rc;
+  ensure_registry();
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (int rc = init_device(dev)) return rc;
+  lp2d_plan* p = new (std::nothrow) lp2d_plan();
+  if (!p) return fail(LP2D_E_ARG, "out of host memory");
+  p->n = n;
+  p->lg = lg;
+  p->batch = batch;
+  p->dtype = dtype;
+  p->device = dev;
+  p->sel.dt = dtype;
+  p->sel.lg = lg;
+  for (int k = 0; k < 3; ++k) p->sel.sel[k] = g_sel[dtype][k][lg];
+  p->sel.chunk = g_chunk[dtype][lg];
+  *out = p;
+  return LP2D_OK;
+}
+
+int lp2d_plan_destroy(lp2d_plan* plan) {
+  delete plan;
+  return LP2D_OK;
+}
+
+int64_t lp2d_plan_workspace_size(const lp2d_plan* p, int op) {
+  if (!p) return -1;
+  size_t a = 0, b = 0;
+  switch (op) {
+    case LP2D_OP_FFT2:
+    case LP2D_OP_PROPAGATE:
+    case LP2D_OP_STACK:
+      return 0;
+    case LP2D_OP_ADJOINT:
+      if (p->dtype == LP2D_C64)
+        adjoint_ws_parts<float>(p->n, p->lg, p->batch, &a, &b);
+      else
+        adjoint_ws_parts<double>(p->n, p->lg, p->batch, &a, &b);
+      return (int64_t)(align256(a) + b);
+    case LP2D_OP_STACK_ADJOINT:
+      return (int64_t)(p->dtype == LP2D_C64 ? stack_adjoint_ws<float>(p->lg, p->batch)
+                                            : stack_adjoint_ws<double>(p->lg, p->batch));
+    default:
+      return -1;
+  }
+}
+
+int lp2d_plan_fft2(const lp2d_plan* p, const void* in, void* out, int inverse, void* stream) {
+  if (int rc = plan_enter(p)) return rc;
+  PlanScope ps(&p->sel);
+  return lp2d_fft2(p->n, p->batch, p->dtype, in, out, inverse, stream);
+}
+
+int lp2d_plan_propagate(const lp2d_plan* p, const void* field_in, void* field_out,
+                        const float* phase, int64_t phase_stride_b, const lp2d_optics* optics,
+                        void* spectrum_out, void* stream) {
+  if (int rc = plan_enter(p)) return rc;
+  PlanScope ps(&p->sel);
+  return lp2d_propagate(p->n, p->batch, p->dtype, field_in, field_out, phase, phase_stride_b,
+                        optics, spectrum_out, stream);
+}
+
+int lp2d_plan_propagate_adjoint(const lp2d_plan* p, const void* grad_out, const void* field_in,
+                                const float* phase, int64_t phase_stride_b,
+                                const lp2d_optics* optics, void* grad_in, float* grad_phase,
+                                int adj_flags, void* work, int64_t work_bytes, void* stream) {
+  if (int rc = plan_enter(p)) return rc;
+  const int n = p->n, batch = p->batch, dtype = p->dtype, lg = p->lg;
+  if (!grad_out || !grad_in || !optics) return fail(LP2D_E_ARG, "null pointer");
+  if (int rc = check_optics(optics)) return rc;
+  if (grad_phase && !phase) return fail(LP2D_E_ARG, "grad_phase requires phase");
+  if ((phase || optics->lens_f != 0.0) && !field_in)
+    return fail(LP2D_E_ARG, "field_in is required with a phase screen or lens");
+  if (field_in && grad_in == field_in) return fail(LP2D_E_ARG, "grad_in must not alias field_in");
+  if (adj_flags & ~LP2D_ADJ_NO_GRAD_IN) return fail(LP2D_E_ARG, "unknown adj_flags bits");
+  size_t plane_b = 0, grad_b = 0;
+  if (dtype == LP2D_C64)
+    adjoint_ws_parts<float>(n, lg, batch, &plane_b, &grad_b);
+  else
+    adjoint_ws_parts<double>(n, lg, batch, &plane_b, &grad_b);
+  const size_t need = align256(plane_b) + grad_b;
+  if (need > 0 && (!work || work_bytes < (int64_t)need))
+    return fail(LP2D_E_SIZE, "n=%d batch=%d: work needs %zu bytes, got %lld", n, batch, need,
+                (long long)work_bytes);
+  if (work && (work == grad_in || work == grad_out || work == field_in ||
+               (grad_phase && work == (void*)grad_phase)))
+    return fail(LP2D_E_ARG, "work must not alias the other arrays");
+  char* w = (char*)work;
+  void* plane = plane_b ? (void*)w : nullptr;
+  float* gplanes = grad_b ? (float*)(w + align256(plane_b)) : nullptr;
+  PlanScope ps(&p->sel);
+  hipStream_t s = (hipStream_t)stream;
+  return dtype == LP2D_C64
+             ? op_adjoint<float>(n, lg, batch, grad_out, field_in, phase, phase_stride_b, optics, grad_in, grad_phase, adj_flags, plane, s, gplanes)
+             : op_adjoint<double>(n, lg, batch, grad_out, field_in, phase, phase_stride_b, optics, grad_in, grad_phase, adj_flags, plane, s, gplanes);
+}
+
+int lp2d_plan_propagate_stack(const lp2d_plan* p, const void* field_in, void* field_out,
+                              const float* screens, int nscreens, const lp2d_optics* optics,
+                              void* const* planes_out, void* stream) {
+  if (int rc = plan_enter(p)) return rc;
+  PlanScope ps(&p->sel);
+  return lp2d_propagate_stack(p->n, p->batch, p->dtype, field_in, field_out, screens, nscreens,
+                              optics, planes_out, stream);
+}
+
+int lp2d_plan_propagate_stack_adjoint(const lp2d_plan* p, const void* grad_out,
+                                      const void* const* planes_in, const float* screens,
+                                      int nscreens, const lp2d_optics* optics, void* grad_in,
+                                      float* grad_screens, int adj_flags, void* work,
+                                      int64_t work_bytes, void* stream) {
+  if (int rc = plan_enter(p)) return rc;
+  PlanScope ps(&p->sel);
+  return lp2d_propagate_stack_adjoint(p->n, p->batch, p->dtype, grad_out, planes_in, screens,
+                                      nscreens, optics, grad_in, grad_screens, adj_flags, work,
+                                      work_bytes, stream);
 }
 
 int lp2d_batch_create(const lp2d_field_desc* descs, int count, int dtype, lp2d_batch** out) {

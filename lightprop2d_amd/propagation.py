@@ -17,7 +17,7 @@ from ._lib import Lp2dError
 
 __all__ = ["fft2", "ifft2", "kz_grid", "spatial_apply", "spectral_apply", "propagate",
            "propagate_adjoint", "propagate_adjoint_noalloc", "propagate_stack",
-           "propagate_stack_adjoint", "intensity", "sum_planes", "UnevenBatch", "optics",
+           "propagate_stack_adjoint", "Plan", "intensity", "sum_planes", "UnevenBatch", "optics",
            "expand", "coarse", "mode_matrix", "modes_project", "modes_construct",
            "Lp2dError"]
 
@@ -412,6 +412,129 @@ def propagate_stack_adjoint(grad_out: torch.Tensor, planes: Sequence[torch.Tenso
             _ptr(gs), flags, _ptr(work), need if work is not None else 0, _stream(grad_out)),
             "propagate_stack_adjoint")
     return (out if want_field_grad else None), gs
+
+
+class Plan:
+    """A plan for fields of one geometry: (batch, n, n) of one complex dtype on one
+    device (lp2d_plan_*).  Creating it initialises the device and copies the kernel
+    schedule selected for n at that moment (later lp2d_set_variant / autotuner
+    changes do not affect it); its calls allocate nothing inside the library, and the
+    adjoint takes a caller-owned workspace (``workspace(op)`` allocates one of
+    ``workspace_size(op)`` bytes).  With a shared screen the plan's adjoint sums the
+    phase gradient over the batch into one (n, n) plane."""
+
+    def __init__(self, n: int, batch: int = 1, dtype=torch.complex64, device=None):
+        lib = _lib()
+        if dtype not in _DT:
+            raise Lp2dError(f"dtype must be complex64 or complex128 (got {dtype})")
+        self.n, self.batch, self.dtype = int(n), int(batch), dtype
+        self.device = (torch.device(device) if device is not None
+                       else torch.device("cuda", torch.cuda.current_device()))
+        self._h = None
+        h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            L.check(lib.lp2d_plan_create(self.n, self.batch, _DT[dtype], C.byref(h)),
+                    "plan_create")
+        self._h = h
+
+    def _handle(self):
+        if self._h is None:
+            raise Lp2dError("plan is closed")
+        return self._h
+
+    def close(self):
+        if self._h is not None and L._lib is not None:
+            L._lib.lp2d_plan_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown
+            pass
+
+    def workspace_size(self, op: int) -> int:
+        b = int(_lib().lp2d_plan_workspace_size(self._handle(), int(op)))
+        if b < 0:
+            raise Lp2dError(f"unknown plan operation {op}")
+        return b
+
+    def workspace(self, op: int) -> Optional[torch.Tensor]:
+        b = self.workspace_size(op)
+        return torch.empty(b, dtype=torch.uint8, device=self.device) if b else None
+
+    def _field(self, t: torch.Tensor, name: str = "field"):
+        n, batch, _ = _field_geom(t, name)
+        if n != self.n or batch != self.batch or t.dtype != self.dtype or \
+                t.device != self.device:
+            raise Lp2dError(f"{name} must be {self.batch} x {self.n}^2 {self.dtype} on "
+                            f"{self.device}; got {tuple(t.shape)} {t.dtype} on {t.device}")
+
+    def fft2(self, field: torch.Tensor, out: Optional[torch.Tensor] = None,
+             inverse: bool = False) -> torch.Tensor:
+        self._field(field)
+        out = torch.empty_like(field) if out is None else out
+        _check_like(out, field, "out")
+        with torch.cuda.device(self.device):
+            L.check(_lib().lp2d_plan_fft2(self._handle(), _ptr(field), _ptr(out), int(inverse),
+                                          _stream(field)), "plan_fft2")
+        return out
+
+    def propagate(self, field: torch.Tensor, z: float, area_size: float, wl: float,
+                  phase: Optional[torch.Tensor] = None, aperture_radius: float = 0.0,
+                  lens_f: float = 0.0, evanescent: str = "reference",
+                  out: Optional[torch.Tensor] = None,
+                  spectrum_out: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """`propagate` with the plan's schedule."""
+        self._field(field)
+        out = torch.empty_like(field) if out is None else out
+        _check_like(out, field, "out")
+        if spectrum_out is not None:
+            _check_like(spectrum_out, field, "spectrum_out")
+        ph, psb = _phase(phase, self.n, self.batch, field.device)
+        o = optics(area_size, wl, z=z, aperture_radius=aperture_radius, lens_f=lens_f,
+                   evanescent=evanescent)
+        with torch.cuda.device(self.device):
+            L.check(_lib().lp2d_plan_propagate(self._handle(), _ptr(field), _ptr(out), _ptr(ph),
+                                               psb, C.byref(o), _ptr(spectrum_out),
+                                               _stream(field)), "plan_propagate")
+        return out
+
+    def propagate_adjoint(self, grad_out: torch.Tensor, field_in: Optional[torch.Tensor],
+                          z: float, area_size: float, wl: float,
+                          phase: Optional[torch.Tensor] = None, aperture_radius: float = 0.0,
+                          lens_f: float = 0.0, evanescent: str = "reference",
+                          want_phase_grad: bool = True, want_field_grad: bool = True,
+                          out: Optional[torch.Tensor] = None,
+                          work: Optional[torch.Tensor] = None):
+        """`propagate_adjoint` with the plan's schedule and a caller workspace (a
+        device tensor of at least workspace_size(LP2D_OP_ADJOINT) bytes; allocated
+        here when not given).  Returns (grad_field, grad_phase)."""
+        self._field(grad_out, "grad_out")
+        out = torch.empty_like(grad_out) if out is None else out
+        _check_like(out, grad_out, "out")
+        if field_in is not None:
+            _check_like(field_in, grad_out, "field_in")
+        ph, psb = _phase(phase, self.n, self.batch, grad_out.device)
+        gph = None
+        if ph is not None and want_phase_grad:
+            shape = (self.n, self.n) if psb == 0 else (self.batch, self.n, self.n)
+            gph = torch.empty(shape, dtype=torch.float32, device=self.device)
+        need = self.workspace_size(L.LP2D_OP_ADJOINT)
+        if work is None and need:
+            work = torch.empty(need, dtype=torch.uint8, device=self.device)
+        if work is not None and (work.device != self.device or not work.is_contiguous()):
+            raise Lp2dError("work must be a contiguous tensor on the plan's device")
+        wb = work.numel() * work.element_size() if work is not None else 0
+        o = optics(area_size, wl, z=z, aperture_radius=aperture_radius, lens_f=lens_f,
+                   evanescent=evanescent)
+        flags = 0 if want_field_grad else L.LP2D_ADJ_NO_GRAD_IN
+        with torch.cuda.device(self.device):
+            L.check(_lib().lp2d_plan_propagate_adjoint(
+                self._handle(), _ptr(grad_out), _ptr(field_in), _ptr(ph), psb, C.byref(o),
+                _ptr(out), _ptr(gph), flags, _ptr(work), wb, _stream(grad_out)),
+                "plan_propagate_adjoint")
+        return (out if want_field_grad else None), gph
 
 
 def propagate_adjoint_noalloc(grad_out: torch.Tensor, field_in: Optional[torch.Tensor],

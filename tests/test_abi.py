@@ -285,3 +285,27 @@ def test_stack_adjoint_checks_arguments_and_workspace(lib):
     assert call(planes, wb=need - 4) == _lib.LP2D_E_SIZE
     assert call(planes, wk=None) == _lib.LP2D_E_SIZE
     assert call(planes, wk=gin) == _lib.LP2D_E_ARG                   # work = grad_in
+
+
+def test_plan_argument_errors(lib):
+    """lp2d_plan_create validates its geometry before touching a device; every plan
+    call rejects a null plan, and the workspace query answers -1 for one."""
+    from lightprop2d_amd import _lib
+    h = C.c_void_p(1234)
+    assert lib.lp2d_plan_create(500, 1, _lib.LP2D_C64, C.byref(h)) == _lib.LP2D_E_SIZE
+    assert h.value is None                       # *out cleared on failure
+    assert lib.lp2d_plan_create(512, 0, _lib.LP2D_C64, C.byref(h)) == _lib.LP2D_E_SIZE
+    assert lib.lp2d_plan_create(512, 1, 7, C.byref(h)) == _lib.LP2D_E_DTYPE
+    assert lib.lp2d_plan_create(512, 1, _lib.LP2D_C64, None) == _lib.LP2D_E_ARG
+    assert lib.lp2d_plan_workspace_size(None, _lib.LP2D_OP_ADJOINT) == -1
+    assert lib.lp2d_plan_destroy(None) == _lib.LP2D_OK
+    p = C.c_void_p(16)
+    o = _lib.lp2d_optics(1.0, 1e-4, 0.1, 0.0, 0.0, 0, 0)
+    assert lib.lp2d_plan_fft2(None, p, p, 0, None) == _lib.LP2D_E_ARG
+    assert lib.lp2d_plan_propagate(None, p, p, None, 0, C.byref(o), None, None) == _lib.LP2D_E_ARG
+    assert lib.lp2d_plan_propagate_adjoint(None, p, p, None, 0, C.byref(o), p, None, 0, None, 0,
+                                           None) == _lib.LP2D_E_ARG
+    assert lib.lp2d_plan_propagate_stack(None, p, p, p, 1, C.byref(o), None,
+                                         None) == _lib.LP2D_E_ARG
+    assert lib.lp2d_plan_propagate_stack_adjoint(None, p, None, p, 1, C.byref(o), p, None, 0,
+                                                 None, 0, None) == _lib.LP2D_E_ARG
