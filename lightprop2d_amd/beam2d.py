@@ -8,7 +8,11 @@ Differences, all deliberate:
 * arrays are ``torch`` tensors on the current HIP device (``xp`` is ``torch``); the
   ``use_gpu`` flag is accepted and ignored (the HIP path always runs);
 * ``complex_bits=64`` computes in complex64 kernels, 128 in complex128 kernels;
-* ``init_field`` is copied to the device rather than aliased;
+* a device tensor ``init_field`` (or ``init_spectrum``) of the beam's complex dtype is
+  kept as the beam's state itself, as the reference keeps the array it is given, so
+  ``coordinate_filter`` / ``lens`` multiply it in place as the reference's
+  ``field *=`` does; a numpy array is copied to the device (as the reference's cupy
+  path copies host arrays) and another dtype is converted;
 * ``field`` is materialised lazily: ``propagate`` and ``spectral_filter`` only multiply
   the carried ``spectrum`` (one elementwise sweep, the transfer function of a
   ``propagate`` is folded into the next spectral operation), and the inverse FFT runs

@@ -254,6 +254,31 @@ def test_beam2d_state_semantics():
     assert "Beam 64x64 points" in repr(b)
 
 
+def test_beam2d_device_init_field_is_aliased():
+    """A device init_field of the beam's dtype becomes the beam's field itself, as the
+    reference keeps the array it is given (beam2d.py:122-129, _asxp :154-187), so
+    coordinate_filter's in-place multiply (`field *=`, :338-341) reaches the caller's
+    tensor.  A host array is copied to the device (the reference's cupy path copies
+    host arrays too) and another dtype is converted, so neither is touched."""
+    from lightprop2d_amd import Beam2D, um, nm
+    n = 64
+    t = torch.ones((n, n), dtype=torch.complex128, device="cuda")
+    b = Beam2D(200 * um, n, 632 * nm, init_field=t)
+    assert b.field is t
+    mask = torch.full((n, n), 0.5, dtype=torch.float64, device="cuda")
+    b.coordinate_filter(f_init=mask)
+    assert b.field is t and torch.equal(t, torch.full_like(t, 0.5))
+    h = np.ones((n, n), dtype=np.complex128)
+    b2 = Beam2D(200 * um, n, 632 * nm, init_field=h)
+    b2.coordinate_filter(f_init=mask)
+    assert np.all(h == 1)
+    t64 = torch.ones((n, n), dtype=torch.complex64, device="cuda")
+    b3 = Beam2D(200 * um, n, 632 * nm, init_field=t64)
+    assert b3.field is not t64 and b3.field.dtype == torch.complex128
+    b3.coordinate_filter(f_init=mask)
+    assert torch.equal(t64, torch.ones_like(t64))
+
+
 def test_beam2d_filters_with_generators():
     """coordinate_filter / spectral_filter with f_gen on the device grids vs the oracle."""
     from lightprop2d_amd import Beam2D, round_hole, um, nm
