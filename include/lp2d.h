@@ -320,6 +320,18 @@ int lp2d_propagate_adjoint_noalloc(int n, int batch, int dtype, const void* grad
                                    void* grad_in, float* grad_phase, int adj_flags,
                                    void* stream);
 
+/* Phase-mask gradient as a separate pass, for callers that compose their own adjoint
+ * from lp2d_fft2 / lp2d_spectral_apply:
+ *   gphi[b] = Im(conj(field[b] * exp(i*phase)) * grad[b])
+ * (the epilogue lp2d_propagate_adjoint fuses into its last sweep; grad is g1 there).
+ * phase may be NULL (no factor); phase_stride_b is 0 (one screen shared by the batch)
+ * or n*n.  sum_batch != 0: grad_phase is one (n, n) float32 plane receiving
+ * sum_b gphi[b]; otherwise (batch, n, n).  No counterpart in the reference (it has no
+ * gradients). */
+int lp2d_phase_grad(int n, int batch, int dtype, const void* field, const void* grad,
+                    const float* phase, int64_t phase_stride_b, float* grad_phase,
+                    int sum_batch, void* stream);
+
 /* Sum `batch` float planes of `count` elements: out[i] = sum_b in[b*count + i].
  * Used to reduce per-field phase gradients of a shared mask. */
 int lp2d_sum_planes(int64_t count, int batch, const float* in, float* out, void* stream);

@@ -44,7 +44,7 @@ EXPORTED = [
     "lp2d_propagate_stack_adjoint", "lp2d_propagate_stack_adjoint_ws_size",
     "lp2d_plan_create", "lp2d_plan_destroy", "lp2d_plan_workspace_size", "lp2d_plan_fft2",
     "lp2d_plan_propagate", "lp2d_plan_propagate_adjoint", "lp2d_plan_propagate_stack",
-    "lp2d_plan_propagate_stack_adjoint",
+    "lp2d_plan_propagate_stack_adjoint", "lp2d_phase_grad",
     "lp2d_sum_planes", "lp2d_batch_create",
     "lp2d_batch_run", "lp2d_batch_destroy", "lp2d_variant_count",
     "lp2d_set_variant", "lp2d_get_variant", "lp2d_variant_name", "lp2d_trace_enable",
@@ -148,6 +148,7 @@ _SIGS = {
                                                C.POINTER(lp2d_optics), _P, _P, _I, _P, _I64,
                                                _P]),
     "lp2d_sum_planes": (_I, [_I64, _I, _P, _P, _P]),
+    "lp2d_phase_grad": (_I, [_I, _I, _I, _P, _P, _P, _I64, _P, _I, _P]),
     "lp2d_batch_create": (_I, [C.POINTER(lp2d_field_desc), _I, _I, C.POINTER(_P)]),
     "lp2d_batch_run": (_I, [_P, _P]),
     "lp2d_batch_destroy": (_I, [_P]),

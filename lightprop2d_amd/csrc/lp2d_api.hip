@@ -413,6 +413,35 @@ __global__ void k_intensity(const cx<T>* __restrict__ f, T* __restrict__ out, lo
   }
 }
 
+// gphi[b, i] = Im(conj(u[b, i] * exp(i*phase)) * g[b, i]) per field, or its sum over
+// the batch (sum != 0: one plane); the screen factor as in the passes (FP64 reduction
+// of phase / 2pi to a fraction of a turn).
+template <typename T>
+__global__ void k_phase_grad(const cx<T>* __restrict__ u, const cx<T>* __restrict__ g,
+                             const float* __restrict__ phase, long long phase_sb,
+                             float* __restrict__ out, long long count, int batch, int sum) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count;
+       i += (long long)gridDim.x * blockDim.x) {
+    float acc = 0.f;
+    cx<T> e = mk((T)1, (T)0);
+    for (int b = 0; b < batch; ++b) {
+      if (phase && (b == 0 || phase_sb != 0)) {
+        double r = (double)phase[(long long)b * phase_sb + i] * kInv2Pi;
+        r -= rint(r);
+        e = exp_i_revolutions<T>(r);
+      }
+      const cx<T> uu = cmul(u[(long long)b * count + i], e);
+      const cx<T> gg = g[(long long)b * count + i];
+      const float v = (float)(uu.re * gg.im - uu.im * gg.re);
+      if (sum)
+        acc += v;
+      else
+        out[(long long)b * count + i] = v;
+    }
+    if (sum) out[i] = acc;
+  }
+}
+
 // out may alias plane 0 of in: each thread reads all planes of its element first.
 __global__ void k_sum_planes(const float* in, float* out, long long count, int batch) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count;
@@ -1174,6 +1203,27 @@ int lp2d_spectral_apply(int n, int batch, int dtype, void* spectrum, const lp2d_
   hipStream_t s = (hipStream_t)stream;
   return dtype == LP2D_C64 ? op_spectral<float>(n, lg, batch, spectrum, masks, nmasks, optics, field_out, s)
                            : op_spectral<double>(n, lg, batch, spectrum, masks, nmasks, optics, field_out, s);
+}
+
+int lp2d_phase_grad(int n, int batch, int dtype, const void* field, const void* grad,
+                    const float* phase, int64_t phase_stride_b, float* grad_phase,
+                    int sum_batch, void* stream) {
+  int lg;
+  if (int rc = check_n(n, batch, dtype, &lg)) return rc;
+  if (!field || !grad || !grad_phase) return fail(LP2D_E_ARG, "null pointer");
+  const long long nn = (long long)n * n;
+  if (phase_stride_b != 0 && phase_stride_b != nn)
+    return fail(LP2D_E_ARG, "phase_stride_b must be 0 or n*n");
+  if ((const void*)grad_phase == field || (const void*)grad_phase == grad)
+    return fail(LP2D_E_ARG, "grad_phase must not alias the fields");
+  hipStream_t s = (hipStream_t)stream;
+  const int sum = sum_batch ? 1 : 0;
+  if (dtype == LP2D_C64)
+    k_phase_grad<float><<<grid_for(nn), 256, 0, s>>>((const cx<float>*)field, (const cx<float>*)grad, phase, phase_stride_b, grad_phase, nn, batch, sum);
+  else
+    k_phase_grad<double><<<grid_for(nn), 256, 0, s>>>((const cx<double>*)field, (const cx<double>*)grad, phase, phase_stride_b, grad_phase, nn, batch, sum);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? LP2D_OK : hip_fail(e, "phase gradient kernel launch");
 }
 
 int lp2d_intensity(int64_t count, int dtype, const void* field, void* out, void* stream) {

@@ -17,7 +17,7 @@ from ._lib import Lp2dError
 
 __all__ = ["fft2", "ifft2", "kz_grid", "spatial_apply", "spectral_apply", "propagate",
            "propagate_adjoint", "propagate_adjoint_noalloc", "propagate_stack",
-           "propagate_stack_adjoint", "Plan", "intensity", "sum_planes", "UnevenBatch", "optics",
+           "propagate_stack_adjoint", "Plan", "phase_grad", "intensity", "sum_planes", "UnevenBatch", "optics",
            "expand", "coarse", "mode_matrix", "modes_project", "modes_construct",
            "Lp2dError"]
 
@@ -574,6 +574,28 @@ def propagate_adjoint_noalloc(grad_out: torch.Tensor, field_in: Optional[torch.T
                                                    _ptr(grad_phase), flags, _stream(grad_out)),
                 "propagate_adjoint_noalloc")
     return out, grad_phase
+
+
+def phase_grad(field: torch.Tensor, grad: torch.Tensor, phase: Optional[torch.Tensor] = None,
+               sum_batch: bool = True, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Im(conj(field * exp(i*phase)) * grad) per field, or summed over the batch
+    (sum_batch, one (n, n) plane): the phase-mask gradient as a separate pass
+    (lp2d_phase_grad), for adjoints composed from fft2 / spectral_apply."""
+    lib = _lib()
+    n, batch, dt = _field_geom(field)
+    _check_like(grad, field, "grad")
+    ph, psb = _phase(phase, n, batch, field.device)
+    shape = (n, n) if sum_batch or field.dim() == 2 else (batch, n, n)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.float32, device=field.device)
+    elif out.shape != shape or out.dtype != torch.float32 or out.device != field.device or \
+            not out.is_contiguous():
+        raise Lp2dError(f"out must be a contiguous float32 tensor of shape {shape}")
+    with torch.cuda.device(field.device):
+        L.check(lib.lp2d_phase_grad(n, batch, dt, _ptr(field), _ptr(grad), _ptr(ph), psb,
+                                    _ptr(out), int(bool(sum_batch)), _stream(field)),
+                "phase_grad")
+    return out
 
 
 def sum_planes(planes: torch.Tensor, out: torch.Tensor):

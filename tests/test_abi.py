@@ -309,3 +309,15 @@ def test_plan_argument_errors(lib):
                                          None) == _lib.LP2D_E_ARG
     assert lib.lp2d_plan_propagate_stack_adjoint(None, p, None, p, 1, C.byref(o), p, None, 0,
                                                  None, 0, None) == _lib.LP2D_E_ARG
+
+
+def test_phase_grad_argument_errors(lib):
+    """lp2d_phase_grad rejects bad geometry, null pointers, a phase stride other than 0
+    or n*n and an output aliasing a field, before any launch."""
+    from lightprop2d_amd import _lib
+    p, q, r = C.c_void_p(16), C.c_void_p(32), C.c_void_p(48)
+    assert lib.lp2d_phase_grad(100, 1, 0, p, q, None, 0, r, 1, None) == _lib.LP2D_E_SIZE
+    assert lib.lp2d_phase_grad(64, 1, 0, None, q, None, 0, r, 1, None) == _lib.LP2D_E_ARG
+    assert lib.lp2d_phase_grad(64, 1, 0, p, q, None, 0, None, 1, None) == _lib.LP2D_E_ARG
+    assert lib.lp2d_phase_grad(64, 2, 0, p, q, p, 7, r, 1, None) == _lib.LP2D_E_ARG
+    assert lib.lp2d_phase_grad(64, 1, 0, p, q, None, 0, p, 1, None) == _lib.LP2D_E_ARG

@@ -467,3 +467,33 @@ def test_stack_adjoint_screen_grad_only_and_errors():
                                           C.c_void_p(gs.data_ptr()), 0,
                                           C.c_void_p(work.data_ptr()), need - 4, None)
     assert rc == L.LP2D_E_SIZE
+
+
+@pytest.mark.parametrize("dt", [torch.complex64, torch.complex128])
+def test_phase_grad_pass_vs_numpy_and_the_fused_adjoint(dt):
+    """lp2d_phase_grad against numpy (per field and batch-summed, shared and per-field
+    screens, no screen), and an adjoint composed from propagate(-z) and phase_grad
+    against lp2d_propagate_adjoint's fused gradient epilogue."""
+    from lightprop2d_amd import propagation as P
+    n, B, area, wl, z = 256, 3, 300e-4, 800e-7, 0.02
+    rs = np.random.RandomState(9)
+    c = np.complex64 if dt == torch.complex64 else np.complex128
+    u = (rs.standard_normal((B, n, n)) + 1j * rs.standard_normal((B, n, n))).astype(c)
+    g = (rs.standard_normal((B, n, n)) + 1j * rs.standard_normal((B, n, n))).astype(c)
+    ph1 = rs.uniform(0, 2 * np.pi, (n, n)).astype(np.float32)
+    phb = rs.uniform(0, 2 * np.pi, (B, n, n)).astype(np.float32)
+    tu, tg = torch.from_numpy(u).to(DEV), torch.from_numpy(g).to(DEV)
+    tol = 1e-5 if dt == torch.complex64 else 1e-6     # float32 output
+    for ph in (None, ph1, phb):
+        e = np.exp(1j * ph.astype(np.float64)) if ph is not None else 1.0
+        ref = np.imag(np.conj(u.astype(np.complex128) * e) * g)
+        tph = torch.from_numpy(ph).to(DEV) if ph is not None else None
+        per = P.phase_grad(tu, tg, tph, sum_batch=False)
+        tot = P.phase_grad(tu, tg, tph, sum_batch=True)
+        assert rel_l2(per.cpu().numpy(), ref) < tol
+        assert rel_l2(tot.cpu().numpy(), ref.sum(axis=0)) < tol
+    tph = torch.from_numpy(ph1).to(DEV)
+    g1 = P.propagate(tg, -z, area, wl)                 # conj(H(z)) = H(-z)
+    composed = P.phase_grad(tu, g1, tph, sum_batch=True)
+    _, fused = P.propagate_adjoint(tg, tu, z, area, wl, phase=tph)
+    assert rel_l2(composed.cpu().numpy(), fused.cpu().numpy()) < tol
