@@ -31,14 +31,14 @@ __global__ __launch_bounds__((N / E) * LINES) LP2D_OCC void k_rows(PassArgs<T> a
 
 // Pipelined row pass: about one resident round of workgroups, each taking every
 // gridDim.x-th row tile of the batch; the twiddle tables are staged in LDS once.
-template <typename T, int N, int E, int LINES, bool INV, int PS, bool TILED>
+template <typename T, int N, int E, int LINES, bool INV, int PS, bool TILED, bool PH = false>
 __global__ __launch_bounds__((N / E) * LINES) LP2D_OCC void k_rows_pipe(PassArgs<T> a, int nunits) {
   __shared__ cx<T> lds[LINES * RowPitch<N, PS, LINES>::value];
   __shared__ cx<T> twl[TwlLen<N, E>::value];
   twl_fill<T, N, E, (N / E) * LINES>(twl, a.tw, threadIdx.x);
   __syncthreads();
-  rows_pipe_body<T, N, E, LINES, INV, PS, TILED>(a, blockIdx.x, gridDim.x, nunits, lds, twl,
-                                                 threadIdx.x);
+  rows_pipe_body<T, N, E, LINES, INV, PS, TILED, PH>(a, blockIdx.x, gridDim.x, nunits, lds, twl,
+                                                     threadIdx.x);
 }
 
 // The pipelined gradient body when its screen factor is hoisted and the LDS twiddle
@@ -232,44 +232,45 @@ hipError_t launch_rows(const PassArgs<T>& a, bool inv, hipStream_t s) {
 // class of the same shape.  (Over an uneven batch's per-field pointer tables, read one
 // tile ahead, it measured slower than the plain class: 2.43 against 2.36 ms for
 // 2048 x 128^2 + 1536 x 256^2 + 512 x 512^2.)
+template <typename T, int N, int E, int LINES, bool INV, int PS, bool TILED, bool PH>
+hipError_t run_rows_pipe(const PassArgs<T>& a, int nu, hipStream_t s) {
+  constexpr int threads = (N / E) * LINES;
+  // same value on every thread that races to set it
+  static std::atomic<int> slot{0};
+  const int sl = cached_slots(slot, (const void*)&k_rows_pipe<T, N, E, LINES, INV, PS, TILED, PH>,
+                              threads);
+  const int grid = nu < sl ? nu : sl;
+  k_rows_pipe<T, N, E, LINES, INV, PS, TILED, PH><<<grid, threads, 0, s>>>(a, nu);
+  return hipGetLastError();
+}
+
 template <typename T, int N, int E, int LINES, int PS>
 hipError_t launch_rows_pipe(const PassArgs<T>& a, bool inv, hipStream_t s) {
   constexpr bool TILE_OK = LINES % tile_rows(N) == 0 && (N / E) % tile_cols(N) == 0;
   if ((a.flags & F_TILED) && !TILE_OK) return hipErrorInvalidValue;
   if constexpr (sizeof(T) == 4 && N % LINES == 0) {
-    const bool feat = (a.flags & (F_SPATIAL | F_GRAD | F_INTENSITY | F_NO_STORE)) != 0;
+    const int fl = a.flags & (F_SPATIAL | F_GRAD | F_INTENSITY | F_NO_STORE);
+    // the forward pass may carry a phase screen alone (shared or per field); at 512
+    // points its prefetched screen values spill (136 bytes per lane), and the plain
+    // pass with the screen prologue runs instead
+    constexpr bool PH_OK = N != 512;
+    const bool ph = PH_OK && !inv && fl == F_PHASE && a.phase;
     const long long units = (long long)a.batch * (N / LINES);
-    if (!feat && !a.src_tab && !a.dst_tab && units <= 0x7fffffff) {
-      constexpr int threads = (N / E) * LINES;
+    if ((fl == 0 || ph) && !a.src_tab && !a.dst_tab && units <= 0x7fffffff) {
       const bool tiled = (a.flags & F_TILED) != 0;
-      const void* k = nullptr;
-      if constexpr (TILE_OK) {
-        if (tiled)
-          k = inv ? (const void*)&k_rows_pipe<T, N, E, LINES, true, PS, true>
-                  : (const void*)&k_rows_pipe<T, N, E, LINES, false, PS, true>;
-      }
-      if (!tiled)
-        k = inv ? (const void*)&k_rows_pipe<T, N, E, LINES, true, PS, false>
-                : (const void*)&k_rows_pipe<T, N, E, LINES, false, PS, false>;
-      // same value on every thread that races to set it
-      static std::atomic<int> slots[4];
-      const int sl = cached_slots(slots[(inv ? 2 : 0) + (tiled ? 1 : 0)], k, threads);
-      const int grid = (int)(units < sl ? units : sl);
       const int nu = (int)units;
       if constexpr (TILE_OK) {
         if (tiled) {
-          if (inv)
-            k_rows_pipe<T, N, E, LINES, true, PS, true><<<grid, threads, 0, s>>>(a, nu);
-          else
-            k_rows_pipe<T, N, E, LINES, false, PS, true><<<grid, threads, 0, s>>>(a, nu);
-          return hipGetLastError();
+          if (inv) return run_rows_pipe<T, N, E, LINES, true, PS, true, false>(a, nu, s);
+          if constexpr (PH_OK)
+            if (ph) return run_rows_pipe<T, N, E, LINES, false, PS, true, true>(a, nu, s);
+          return run_rows_pipe<T, N, E, LINES, false, PS, true, false>(a, nu, s);
         }
       }
-      if (inv)
-        k_rows_pipe<T, N, E, LINES, true, PS, false><<<grid, threads, 0, s>>>(a, nu);
-      else
-        k_rows_pipe<T, N, E, LINES, false, PS, false><<<grid, threads, 0, s>>>(a, nu);
-      return hipGetLastError();
+      if (inv) return run_rows_pipe<T, N, E, LINES, true, PS, false, false>(a, nu, s);
+      if constexpr (PH_OK)
+        if (ph) return run_rows_pipe<T, N, E, LINES, false, PS, false, true>(a, nu, s);
+      return run_rows_pipe<T, N, E, LINES, false, PS, false, false>(a, nu, s);
     }
   }
   return launch_rows<T, N, E, LINES, PS>(a, inv, s);

@@ -756,9 +756,11 @@ __device__ __forceinline__ void rows_adj_chain_body(const PassArgs<T>& a, int b0
 // workgroup takes tiles u0, u0 + ustep, u0 + 2*ustep, ... of the whole batch, so the
 // resident workgroups sweep the planes together.  Twiddles come from LDS (twl), so
 // nothing in the loop waits for the prefetch before it is needed.  Strided batches
-// of complex64 without spatial features; TILED: the forward pass stores, the inverse
-// pass loads the block-tiled layout.
-template <typename T, int N, int E, int LINES, bool INV, int PS, bool TILED>
+// of complex64 without spatial features, except (PH, forward pass) a phase screen:
+// its values are prefetched with the tile and applied before the next prefetch is
+// issued.  TILED: the forward pass stores, the inverse pass loads the block-tiled
+// layout.
+template <typename T, int N, int E, int LINES, bool INV, int PS, bool TILED, bool PH = false>
 __device__ __forceinline__ void rows_pipe_body(const PassArgs<T>& a, int u0, int ustep,
                                                int nunits, cx<T>* lds, const cx<T>* twl,
                                                int tid) {
@@ -802,16 +804,34 @@ __device__ __forceinline__ void rows_pipe_body(const PassArgs<T>& a, int u0, int
   // Step u moves tile u (prefetched by the previous step) from nxt into cur, requests
   // tile u + ustep into nxt, then transforms cur and stores it; the first step is
   // peeled so that the loop head's wait only ever covers the prefetch.
+  static_assert(!PH || !INV, "the screen is a forward-pass prologue");
   cx<T> cur[E], nxt[E];
+  float phn[PH ? E : 1];
   auto fetch = [&](int u) {
     const LP2D_GLB cx<T>* __restrict__ src = glb(uniform_ptr(a.src + tile_off(u)));
 #pragma unroll
     for (int m = 0; m < E; ++m)
       nxt[m] = gld<LD_T ? NT_ROW_LD : nt_row_site<T, TL>()>(src + (ld_o + LD_S * (unsigned)m));
+    if constexpr (PH) {
+      const int b = u / TPF;
+      const LP2D_GLB float* __restrict__ ph =
+          glb(uniform_ptr(a.phase + (long long)b * a.phase_sb +
+                          (long long)(u - b * TPF) * LINES * N)) + lo_n;
+#pragma unroll
+      for (int m = 0; m < E; ++m) phn[m] = ph[TL * m];
+    }
   };
   auto step = [&](int u) {
 #pragma unroll
     for (int m = 0; m < E; ++m) cur[m] = nxt[m];
+    if constexpr (PH) {  // x exp(i*phase): FP64 reduction to a fraction of a turn
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        double r = (double)phn[m] * kInv2Pi;
+        r -= rint(r);
+        cur[m] = cmul(cur[m], exp_i_revolutions<T>(r));
+      }
+    }
     if (u + ustep < nunits) fetch(u + ustep);  // uniform branch
     line_fft_l<T, N, E, INV>(cur, t, acc, twl);
     if (scale) {

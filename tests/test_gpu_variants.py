@@ -27,8 +27,10 @@ OPTICS = {"pupil": (400e-4, 632e-7, 50e-4, 0.2), "evanescent": (6e-4, 632e-7, 3e
 
 
 def _lib():
-    from lightprop2d_amd import _lib as L
-    return L.lib()
+    # through propagation, which installs the tuned selection on first use, so that
+    # restore_selection saves (and puts back) the selection every other test runs with
+    from lightprop2d_amd import propagation as P
+    return P._lib()
 
 
 def _names(lib, n, dt, kind):
@@ -82,6 +84,37 @@ def test_every_variant_matches_oracle(restore_selection, n, dtype, which):
             if spec is not None:
                 assert rel_l2(spec.cpu().numpy(), spec_exp) < tol, name
         lib.lp2d_set_variant(n, dt, kind, default)
+
+
+@pytest.mark.parametrize("n", [256, 512, 1024, 2048, 4096])
+def test_every_row_variant_with_a_phase_screen(restore_selection, n):
+    """The forward row pass with a phase screen (the prologue of lp2d_propagate with
+    `phase`): every row variant of complex64 n, a screen shared by the batch and one
+    per field, against the oracle.  The pipelined class carries the screen itself
+    (prefetched with the tile) except at 512 points, where the plain class runs."""
+    from lightprop2d_amd import propagation as P
+    lib = restore_selection
+    area, wl, z, na = OPTICS["pupil"]
+    batch = 3 if n <= 1024 else (2 if n <= 2048 else 1)
+    rs = np.random.RandomState(3 * n)
+    u = (rs.standard_normal((batch, n, n)) + 1j * rs.standard_normal((batch, n, n))).astype(
+        np.complex64)
+    shared = rs.uniform(0, 2 * np.pi, (n, n)).astype(np.float32)
+    per = rs.uniform(0, 2 * np.pi, (batch, n, n)).astype(np.float32)
+    tu = torch.from_numpy(u).to(DEV)
+    default = lib.lp2d_get_variant(n, 0, 0)
+    for ph in (shared, per):
+        exp = np.stack([oracle.propagate_field(u[i], area, wl, z, aperture_radius=na / wl,
+                                               phase_mask=(ph if ph.ndim == 2 else ph[i])
+                                               .astype(np.float64))
+                        for i in range(batch)])
+        tph = torch.from_numpy(ph).to(DEV)
+        for v, name in enumerate(_names(lib, n, 0, 0)):
+            print(f"n={n} screen {ph.shape} row variant={v} [{name}]", flush=True)
+            assert lib.lp2d_set_variant(n, 0, 0, v) == 0
+            out = P.propagate(tu, z, area, wl, phase=tph, aperture_radius=na / wl)
+            assert rel_l2(out.cpu().numpy(), exp) < 1e-5, name
+    lib.lp2d_set_variant(n, 0, 0, default)
 
 
 @pytest.mark.parametrize("kind", [0, 1], ids=["rows", "cols"])
