@@ -152,6 +152,7 @@ def extra_legs(torch, P, L, lib, dev):
     out = {"uneven_batch": uneven_leg(torch, P, L, lib, dev),
            "speckle_chain": speckle_leg(torch, P, dev),
            "beam2d_chain": beam2d_leg(torch, P, dev),
+           "published": published_leg(torch, dev),
            "stack": stack_leg(torch, P, L, lib, dev)}
     out["large_fields"] = large_field_leg(torch, P, L, lib, dev)
     out["generators"] = generator_leg(torch, dev)
@@ -365,6 +366,58 @@ def stack_leg(torch, P, L, lib, dev):
     return res
 
 
+def published_leg(torch, dev):
+    """The reference's own published timings (the %timeit transcripts of
+    Beam2D._update_obj's docstring, beam2d.py:236-269, 1024^2 complex128, the default
+    precision; old micrometre API Beam2D(200, 1024, 0.632, ...) = area 200 um, wl
+    632 nm, Gaussian rho0 50 um), repeated through the drop-in Beam2D: construction
+    from gaussian_beam, from an array, from an array and a spectrum, and _update_obj
+    with and without a spectrum.  Each call is timed from the host to the end of its
+    GPU work (synchronised, `us`) and, as %timeit on CuPy measures, without waiting
+    (`async_us`); both are means over repeated calls.  The published figures are from
+    an unstated NVIDIA GPU (CuPy 11.2) and an unstated CPU."""
+    from lightprop2d_amd import Beam2D, gaussian_beam, um, nm
+    area, n, wl = 200 * um, 1024, 632 * nm
+    b = Beam2D(area, n, wl, init_field_gen=gaussian_beam, init_gen_args=(1, 50 * um))
+    a = Beam2D(area, n, wl, init_field=b.field)
+    ops = {
+        "construct_from_gaussian_beam": (
+            lambda: Beam2D(area, n, wl, init_field_gen=gaussian_beam, init_gen_args=(1, 50 * um)),
+            2750.0, 81000.0),
+        "construct_from_array": (lambda: Beam2D(area, n, wl, init_field=b.field), 2160.0, 55800.0),
+        "update_obj_field": (lambda: a._update_obj(b.field), 66.6, 36400.0),
+        "construct_with_spectrum": (
+            lambda: Beam2D(area, n, wl, init_field=b.field, init_spectrum=b.spectrum),
+            1100.0, 17200.0),
+        "update_obj_field_spectrum": (lambda: a._update_obj(b.field, spectrum=b.spectrum),
+                                      1.39, 1.12),
+    }
+    out = {}
+    for name, (fn, pub_gpu, pub_cpu) in ops.items():
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        reps = 50
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+            torch.cuda.synchronize()
+        sync_us = (time.perf_counter() - t0) / reps * 1e6
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        async_us = (time.perf_counter() - t0) / reps * 1e6
+        torch.cuda.synchronize()
+        out[name] = {"us": round(sync_us, 2), "async_us": round(async_us, 2),
+                     "published_gpu_us": pub_gpu, "published_cpu_us": pub_cpu,
+                     "speedup_vs_published_gpu": round(pub_gpu / sync_us, 2),
+                     "speedup_vs_published_gpu_async": round(pub_gpu / async_us, 2)}
+    out["workload"] = ("the reference's published Beam2D timings at 1024^2 complex128 "
+                       "(beam2d.py:236-269), through the drop-in Beam2D")
+    return out
+
+
 def beam2d_leg(torch, P, dev):
     """Configuration 2 through the drop-in Beam2D: 1024^2 complex64 speckle field,
     10 x [propagate(50 um), spectral_filter(NA 0.2 pupil array)], then iprofile on the
@@ -534,6 +587,7 @@ LEGS = {
     "uneven_batch": lambda torch, P, L, lib, dev: uneven_leg(torch, P, L, lib, dev),
     "speckle_chain": lambda torch, P, L, lib, dev: speckle_leg(torch, P, dev),
     "beam2d_chain": lambda torch, P, L, lib, dev: beam2d_leg(torch, P, dev),
+    "published": lambda torch, P, L, lib, dev: published_leg(torch, dev),
     "stack": lambda torch, P, L, lib, dev: stack_leg(torch, P, L, lib, dev),
     "large_fields": lambda torch, P, L, lib, dev: large_field_leg(torch, P, L, lib, dev),
     "resample": lambda torch, P, L, lib, dev: resample_leg(torch, P, dev),
