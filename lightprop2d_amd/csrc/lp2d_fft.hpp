@@ -297,8 +297,26 @@ __device__ __forceinline__ void twl_fill(cx<T>* twl, const cx<T>* __restrict__ t
   }
 }
 
-// The Stockham passes above with the twiddles read from the LDS tables.
-template <typename T, int N, int E, bool INV, int NS, bool CL = false, typename Acc>
+// Synchronisation of an exchange among lanes of one wave: LDS accesses of a wave
+// execute in order, so ordering the compiler's accesses is enough (no workgroup
+// barrier, other waves run on).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+template <bool WS>
+__device__ __forceinline__ void exchange_sync() {
+  if constexpr (WS)
+    wave_sync();
+  else
+    __syncthreads();
+}
+
+// The Stockham passes above with the twiddles read from the LDS tables.  WS: every
+// line's TL = N/E lanes lie in one wave, so the exchanges synchronise that wave only.
+template <typename T, int N, int E, bool INV, int NS, bool CL = false, bool WS = false,
+          typename Acc>
 __device__ __forceinline__ void stockham_l(cx<T> (&v)[E], int t, const Acc& acc,
                                            const cx<T>* twl) {
   constexpr int REM = N / NS;
@@ -337,18 +355,19 @@ __device__ __forceinline__ void stockham_l(cx<T> (&v)[E], int t, const Acc& acc,
     }
   }
   if constexpr (!LAST) {
-    __syncthreads();
+    exchange_sync<WS>();
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = acc.ld(t + TL * m);
-    __syncthreads();
-    stockham_l<T, N, E, INV, NS * R, CL>(v, t, acc, twl);
+    exchange_sync<WS>();
+    stockham_l<T, N, E, INV, NS * R, CL, WS>(v, t, acc, twl);
   }
 }
 
-template <typename T, int N, int E, bool INV, bool CL = false, typename Acc>
+template <typename T, int N, int E, bool INV, bool CL = false, bool WS = false, typename Acc>
 __device__ __forceinline__ void line_fft_l(cx<T> (&v)[E], int t, const Acc& acc,
                                            const cx<T>* twl) {
-  stockham_l<T, N, E, INV, 1, CL>(v, t, acc, twl);
+  static_assert(!WS || 64 % (N / E) == 0, "a line's lanes within one wave");
+  stockham_l<T, N, E, INV, 1, CL, WS>(v, t, acc, twl);
 }
 
 }  // namespace lp2d

@@ -1422,8 +1422,14 @@ __device__ __forceinline__ void small_pipe_body(const PassArgs<T>& a, int b0, in
   const bool mid = (flags & (F_H | F_APERTURE)) != 0;
   const bool scale = (flags & F_SCALE) != 0;
   const bool spec_out = (flags & F_SPEC_OUT) != 0;
-  const int y = tid / TL, tr = tid % TL;  // row phases: row y, lane tr
-  const int x = tid % N, tc = tid / N;    // column phase: column x, lane tc
+  // row phases: row y, lane tr (a row's TL lanes are consecutive, within one wave);
+  // column phase: column x, lane tc, each wave taking CPW whole columns, so every
+  // exchange of a line transform stays inside one wave (no workgroup barrier)
+  constexpr int THREADS = N * TL;
+  constexpr int CPW = (THREADS < 64 ? THREADS : 64) / TL;
+  static_assert(64 % TL == 0 && (THREADS < 64 || THREADS % 64 == 0), "whole lines per wave");
+  const int y = tid / TL, tr = tid % TL;
+  const int x = (tid / 64) * CPW + tid % CPW, tc = (tid % 64) / CPW;
   const int sx = x < N / 2 ? x : x - N;
   cx<T> cur[E], nxt[E];
   auto fetch = [&](int bf) {
@@ -1438,7 +1444,7 @@ __device__ __forceinline__ void small_pipe_body(const PassArgs<T>& a, int b0, in
     if (b + 1 < b1) fetch(b + 1);  // uniform branch
     {  // rows: forward FFT into the tile
       LineAcc<T, PS> acc{lds + y * P};
-      line_fft_l<T, N, E, false>(cur, tr, acc, twl);
+      line_fft_l<T, N, E, false, false, true>(cur, tr, acc, twl);
 #pragma unroll
       for (int m = 0; m < E; ++m) acc.st(tr + TL * m, cur[m]);
     }
@@ -1448,8 +1454,8 @@ __device__ __forceinline__ void small_pipe_body(const PassArgs<T>& a, int b0, in
       cx<T> v[E];
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = acc.ld(tc + TL * m);
-      __syncthreads();  // the column FFT below overwrites rows other threads still read
-      line_fft_l<T, N, E, false>(v, tc, acc, twl);
+      wave_sync();  // the column's own lanes read it before its transform rewrites it
+      line_fft_l<T, N, E, false, false, true>(v, tc, acc, twl);
       if (mid) {
         ColMid cm = col_mid(pf[b - b0].o, flags, sx);
         int tt = tc;
@@ -1462,8 +1468,8 @@ __device__ __forceinline__ void small_pipe_body(const PassArgs<T>& a, int b0, in
 #pragma unroll
         for (int m = 0; m < E; ++m) gst<NT_SMALL>(so + ((long long)(tc + TL * m) * N), v[m]);
       }
-      line_fft_l<T, N, E, true>(v, tc, acc, twl);
-      __syncthreads();
+      line_fft_l<T, N, E, true, false, true>(v, tc, acc, twl);
+      wave_sync();
 #pragma unroll
       for (int m = 0; m < E; ++m) acc.st(tc + TL * m, v[m]);
     }
@@ -1473,8 +1479,8 @@ __device__ __forceinline__ void small_pipe_body(const PassArgs<T>& a, int b0, in
       cx<T> v[E];
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = acc.ld(tr + TL * m);
-      __syncthreads();
-      line_fft_l<T, N, E, true>(v, tr, acc, twl);
+      wave_sync();  // the row's own lanes read it before its transform rewrites it
+      line_fft_l<T, N, E, true, false, true>(v, tr, acc, twl);
       if (scale) {
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = cscale(v[m], a.scale);
