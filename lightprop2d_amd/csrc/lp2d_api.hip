@@ -1375,8 +1375,10 @@ int lp2d_propagate_stack_adjoint(int n, int batch, int dtype, const void* grad_o
     if (!work || work_bytes < (int64_t)need)
       return fail(LP2D_E_SIZE, "n=%d batch=%d: work needs %zu bytes, got %lld", n, batch, need,
                   (long long)work_bytes);
-    if (work == grad_in || work == grad_out || work == grad_screens)
-      return fail(LP2D_E_ARG, "work must not alias grad_out, grad_in or grad_screens");
+    if (work == grad_in || work == grad_out || work == grad_screens || work == screens)
+      return fail(LP2D_E_ARG, "work must not alias grad_out, grad_in, screens or grad_screens");
+    for (int k = 0; k < nscreens; ++k)
+      if (work == planes_in[k]) return fail(LP2D_E_ARG, "work must not alias planes_in[%d]", k);
   }
   hipStream_t s = (hipStream_t)stream;
   float* ws = grad_screens && need > 0 ? (float*)work : nullptr;

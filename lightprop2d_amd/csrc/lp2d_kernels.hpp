@@ -610,7 +610,9 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
   LP2D_GLB cx<T>* __restrict__ dst = field_dst(a, b) + ubase;
   // whole lines per store instruction: natural map / tiled-plane map
   constexpr bool WN = nt_store_site<T>(1, TL) != 0;
-  constexpr bool WT = nt_store_site<T>(1, tiled_map_run<N, TL, LINES>()) != 0;
+  constexpr bool WT =
+      nt_store_site<T>(1, (INV && TiledRowMap<N, TL, LINES>::ok) ? tiled_map_run<N, TL, LINES>()
+                                                                  : TL) != 0;
   if constexpr (FEAT) {
     spatial_epi_all<T, E>(a, v, b, y, t, TL, N, frow + t, valid);
     if ((a.flags & F_NO_STORE) == 0) {

@@ -285,6 +285,8 @@ def test_stack_adjoint_checks_arguments_and_workspace(lib):
     assert call(planes, wb=need - 4) == _lib.LP2D_E_SIZE
     assert call(planes, wk=None) == _lib.LP2D_E_SIZE
     assert call(planes, wk=gin) == _lib.LP2D_E_ARG                   # work = grad_in
+    assert call(planes, wk=C.c_void_p(96)) == _lib.LP2D_E_ARG        # work = u_0
+    assert call(planes, wk=scr) == _lib.LP2D_E_ARG                   # work = screens
 
 
 def test_plan_argument_errors(lib):
