@@ -186,6 +186,22 @@ struct StrideAcc {
   __device__ __forceinline__ cx<T> ld(int i) const { return base[i * P]; }
 };
 
+// Synchronisation of an exchange among lanes of one wave: LDS accesses of a wave
+// execute in order, so ordering the compiler's accesses is enough (no workgroup
+// barrier, other waves run on).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+template <bool WS>
+__device__ __forceinline__ void exchange_sync() {
+  if constexpr (WS)
+    wave_sync();
+  else
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // Stockham autosort passes.  Pass with stride NS (product of the previous radices)
 // and radix R: butterfly j in [0, N/R) reads in[j + r*N/R], multiplies element r by
@@ -196,7 +212,7 @@ struct StrideAcc {
 //
 // tw points at the twiddle table of length N for this direction:
 // tw[i] = exp(-+2*pi*i*i/N) (minus for the forward transform).
-template <typename T, int N, int E, bool INV, int NS, typename Acc>
+template <typename T, int N, int E, bool INV, int NS, bool WS = false, typename Acc>
 __device__ __forceinline__ void stockham(cx<T> (&v)[E], int t, const Acc& acc,
                                          const cx<T>* __restrict__ tw) {
   constexpr int REM = N / NS;
@@ -236,19 +252,22 @@ __device__ __forceinline__ void stockham(cx<T> (&v)[E], int t, const Acc& acc,
     }
   }
   if constexpr (!LAST) {
-    __syncthreads();
+    exchange_sync<WS>();
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = acc.ld(t + TL * m);
-    __syncthreads();
-    stockham<T, N, E, INV, NS * R>(v, t, acc, tw);
+    exchange_sync<WS>();
+    stockham<T, N, E, INV, NS * R, WS>(v, t, acc, tw);
   }
 }
 
-template <typename T, int N, int E, bool INV, typename Acc>
+// WS: every line's TL = N/E lanes lie in one wave, so the exchanges synchronise that
+// wave only.
+template <typename T, int N, int E, bool INV, bool WS = false, typename Acc>
 __device__ __forceinline__ void line_fft(cx<T> (&v)[E], int t, const Acc& acc,
                                          const cx<T>* __restrict__ tw) {
   static_assert((N & (N - 1)) == 0 && (E & (E - 1)) == 0 && E <= N, "powers of two");
-  stockham<T, N, E, INV, 1>(v, t, acc, tw);
+  static_assert(!WS || 64 % (N / E) == 0, "a line's lanes within one wave");
+  stockham<T, N, E, INV, 1, WS>(v, t, acc, tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -295,22 +314,6 @@ __device__ __forceinline__ void twl_fill(cx<T>* twl, const cx<T>* __restrict__ t
     }
     twl_fill<T, N, E, THREADS, NS * R, CL>(twl, tw, tid);
   }
-}
-
-// Synchronisation of an exchange among lanes of one wave: LDS accesses of a wave
-// execute in order, so ordering the compiler's accesses is enough (no workgroup
-// barrier, other waves run on).
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-template <bool WS>
-__device__ __forceinline__ void exchange_sync() {
-  if constexpr (WS)
-    wave_sync();
-  else
-    __syncthreads();
 }
 
 // The Stockham passes above with the twiddles read from the LDS tables.  WS: every

@@ -606,7 +606,13 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
     }
   }
   LineAcc<T, PS> acc{lds + l * L};
-  line_fft<T, N, E, INV>(v, t, acc, (INV ? a.tw_inv : a.tw) + (N - 2));
+  // each row's TL lanes in one wave, with the natural map and with the tiled-plane map
+  // of the inverse pass: the transform's exchanges then synchronise one wave
+  using RMAP = TiledRowMap<N, TL, LINES>;
+  constexpr bool NAT1 = TL <= 64 && 64 % TL == 0;
+  constexpr bool TIL1 = RMAP::LINE16 ? TL == 16 : RMAP::WM::ok && TL == 64 / TR;
+  constexpr bool WS = NAT1 && (!(INV && RMAP::ok) || TIL1);
+  line_fft<T, N, E, INV, WS>(v, t, acc, (INV ? a.tw_inv : a.tw) + (N - 2));
   LP2D_GLB cx<T>* __restrict__ dst = field_dst(a, b) + ubase;
   // whole lines per store instruction: natural map / tiled-plane map
   constexpr bool WN = nt_store_site<T>(1, TL) != 0;
