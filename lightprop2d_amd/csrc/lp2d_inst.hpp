@@ -31,14 +31,15 @@ __global__ __launch_bounds__((N / E) * LINES) LP2D_OCC void k_rows(PassArgs<T> a
 
 // Pipelined row pass: about one resident round of workgroups, each taking every
 // gridDim.x-th row tile of the batch; the twiddle tables are staged in LDS once.
-template <typename T, int N, int E, int LINES, bool INV, int PS, bool TILED, bool PH = false>
+template <typename T, int N, int E, int LINES, bool INV, int PS, bool TILED, bool PH = false,
+          bool WSK = true>
 __global__ __launch_bounds__((N / E) * LINES) LP2D_OCC void k_rows_pipe(PassArgs<T> a, int nunits) {
   __shared__ cx<T> lds[LINES * RowPitch<N, PS, LINES>::value];
   __shared__ cx<T> twl[TwlLen<N, E>::value];
   twl_fill<T, N, E, (N / E) * LINES>(twl, a.tw, threadIdx.x);
   __syncthreads();
-  rows_pipe_body<T, N, E, LINES, INV, PS, TILED, PH>(a, blockIdx.x, gridDim.x, nunits, lds, twl,
-                                                     threadIdx.x);
+  rows_pipe_body<T, N, E, LINES, INV, PS, TILED, PH, WSK>(a, blockIdx.x, gridDim.x, nunits, lds,
+                                                          twl, threadIdx.x);
 }
 
 // The pipelined gradient body when its screen factor is hoisted and the LDS twiddle
@@ -232,16 +233,28 @@ hipError_t launch_rows(const PassArgs<T>& a, bool inv, hipStream_t s) {
 // class of the same shape.  (Over an uneven batch's per-field pointer tables, read one
 // tile ahead, it measured slower than the plain class: 2.43 against 2.36 ms for
 // 2048 x 128^2 + 1536 x 256^2 + 512 x 512^2.)
-template <typename T, int N, int E, int LINES, bool INV, int PS, bool TILED, bool PH>
-hipError_t run_rows_pipe(const PassArgs<T>& a, int nu, hipStream_t s) {
+template <typename T, int N, int E, int LINES, bool INV, int PS, bool TILED, bool PH,
+          bool WSK = true>
+hipError_t run_rows_pipe_k(const PassArgs<T>& a, int nu, hipStream_t s) {
   constexpr int threads = (N / E) * LINES;
   // same value on every thread that races to set it
   static std::atomic<int> slot{0};
-  const int sl = cached_slots(slot, (const void*)&k_rows_pipe<T, N, E, LINES, INV, PS, TILED, PH>,
-                              threads);
+  const int sl = cached_slots(
+      slot, (const void*)&k_rows_pipe<T, N, E, LINES, INV, PS, TILED, PH, WSK>, threads);
   const int grid = nu < sl ? nu : sl;
-  k_rows_pipe<T, N, E, LINES, INV, PS, TILED, PH><<<grid, threads, 0, s>>>(a, nu);
+  k_rows_pipe<T, N, E, LINES, INV, PS, TILED, PH, WSK><<<grid, threads, 0, s>>>(a, nu);
   return hipGetLastError();
+}
+// A pass converting between the natural and the block-tiled layout in place keeps the
+// workgroup-synchronised transform (the wave-local one would let a wave store over
+// data another wave has yet to load).
+template <typename T, int N, int E, int LINES, bool INV, int PS, bool TILED, bool PH>
+hipError_t run_rows_pipe(const PassArgs<T>& a, int nu, hipStream_t s) {
+  if constexpr (TILED) {
+    if ((const void*)a.src == (const void*)a.dst)
+      return run_rows_pipe_k<T, N, E, LINES, INV, PS, TILED, PH, false>(a, nu, s);
+  }
+  return run_rows_pipe_k<T, N, E, LINES, INV, PS, TILED, PH, true>(a, nu, s);
 }
 
 template <typename T, int N, int E, int LINES, int PS>

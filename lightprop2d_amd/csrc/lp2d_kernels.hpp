@@ -606,13 +606,7 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
     }
   }
   LineAcc<T, PS> acc{lds + l * L};
-  // each row's TL lanes in one wave, with the natural map and with the tiled-plane map
-  // of the inverse pass: the transform's exchanges then synchronise one wave
-  using RMAP = TiledRowMap<N, TL, LINES>;
-  constexpr bool NAT1 = TL <= 64 && 64 % TL == 0;
-  constexpr bool TIL1 = RMAP::LINE16 ? TL == 16 : RMAP::WM::ok && TL == 64 / TR;
-  constexpr bool WS = NAT1 && (!(INV && RMAP::ok) || TIL1);
-  line_fft<T, N, E, INV, WS>(v, t, acc, (INV ? a.tw_inv : a.tw) + (N - 2));
+  line_fft<T, N, E, INV>(v, t, acc, (INV ? a.tw_inv : a.tw) + (N - 2));
   LP2D_GLB cx<T>* __restrict__ dst = field_dst(a, b) + ubase;
   // whole lines per store instruction: natural map / tiled-plane map
   constexpr bool WN = nt_store_site<T>(1, TL) != 0;
@@ -795,7 +789,8 @@ __device__ __forceinline__ void rows_adj_chain_body(const PassArgs<T>& a, int b0
 // its values are prefetched with the tile and applied before the next prefetch is
 // issued.  TILED: the forward pass stores, the inverse pass loads the block-tiled
 // layout.
-template <typename T, int N, int E, int LINES, bool INV, int PS, bool TILED, bool PH = false>
+template <typename T, int N, int E, int LINES, bool INV, int PS, bool TILED, bool PH = false,
+          bool WSK = true>
 __device__ __forceinline__ void rows_pipe_body(const PassArgs<T>& a, int u0, int ustep,
                                                int nunits, cx<T>* lds, const cx<T>* twl,
                                                int tid) {
@@ -834,6 +829,15 @@ __device__ __forceinline__ void rows_pipe_body(const PassArgs<T>& a, int u0, int
   constexpr int ST_RUN = (INV && TILED) ? ((TR == 8 && TC == 8 && TL % 16 == 0) ? 16
                                            : WM::ok ? 64 / TR : TL)
                                         : TL;
+  // each row's TL lanes in one wave (then the transform's exchanges synchronise one
+  // wave): the natural map up to 64 lanes, the tiled-plane maps when a wave holds
+  // whole rows.  Not for a layout conversion in place (WSK false): there the
+  // workgroup barriers inside the transform are what keep one wave's stores off
+  // data another wave of the workgroup has yet to load.
+  constexpr bool WS = WSK && TL <= 64 && 64 % TL == 0 &&
+                      (!(INV && TILED) ||
+                       ((TR == 8 && TC == 8 && TL % 16 == 0) ? TL == 16
+                        : WM::ok ? TL == 64 / TR : true));
   LineAcc<T, PS> acc{lds + l * L};
   const bool scale = (a.flags & F_SCALE) != 0;
   auto tile_off = [&](int u) {
@@ -872,7 +876,7 @@ __device__ __forceinline__ void rows_pipe_body(const PassArgs<T>& a, int u0, int
       }
     }
     if (u + ustep < nunits) fetch(u + ustep);  // uniform branch
-    line_fft_l<T, N, E, INV>(cur, t, acc, twl);
+    line_fft_l<T, N, E, INV, false, WS>(cur, t, acc, twl);
     if (scale) {
 #pragma unroll
       for (int m = 0; m < E; ++m) cur[m] = cscale(cur[m], a.scale);
