@@ -153,6 +153,7 @@ def extra_legs(torch, P, L, lib, dev):
            "speckle_chain": speckle_leg(torch, P, dev),
            "beam2d_chain": beam2d_leg(torch, P, dev),
            "published": published_leg(torch, dev),
+           "phase_mask_design": phase_mask_leg(torch, dev),
            "stack": stack_leg(torch, P, L, lib, dev)}
     out["large_fields"] = large_field_leg(torch, P, L, lib, dev)
     out["generators"] = generator_leg(torch, dev)
@@ -364,6 +365,87 @@ def stack_leg(torch, P, L, lib, dev):
                        "adjoints, and one checkpointed gradient step (loss sum(w |u_5|^2))")
     del u, scr, out, tmp, sr, w
     return res
+
+
+def phase_mask_leg(torch, dev):
+    """Configuration 4: a 512^2 float32 phase mask (seeded U[0, 2pi)) on a Gaussian
+    input (rho0 50 um, area 500 um, 1064 nm): propagate 1 mm through the mask, lens
+    f = 5 mm (reference convention) and propagate 5 mm, MSE of the intensity against a
+    ring (radius 30 um, width 5 um); 200 Adam steps (lr 0.1) through the hand-written
+    adjoint (lightprop2d_amd.autograd).  One field per step, so each step is bound by
+    launch latency; reported as ms per step (forward, loss, backward, optimiser)."""
+    from lightprop2d_amd import autograd as AG
+    from lightprop2d_amd import profiles as PP
+    n, area, wl, R = 512, 500e-4, 1064e-7, 50e-4
+    x = (np.arange(n) - n // 2) * area / n
+    u0 = PP.gaussian_beam(x.copy(), x.reshape(-1, 1).copy(), 1.0, R)
+    r = np.sqrt(x ** 2 + x.reshape(-1, 1) ** 2)
+    target = torch.from_numpy(np.exp(-(r - 30e-4) ** 2 / (2 * (5e-4) ** 2)).astype(np.float32)).to(dev)
+    u0_t = torch.from_numpy(np.asarray(u0).astype(np.complex64)).to(dev)
+    phi = torch.from_numpy(np.random.RandomState(0).uniform(0, 2 * np.pi, (n, n))
+                           .astype(np.float32)).to(dev).requires_grad_(True)
+    opt = torch.optim.Adam([phi], lr=0.1)
+
+    def step():
+        opt.zero_grad()
+        u1 = AG.propagate(u0_t, 1e-1, area, wl, phase=phi)
+        u2 = AG.propagate(u1, 5e-1, area, wl, lens_f=5e-1)
+        loss = ((u2.real ** 2 + u2.imag ** 2 - target) ** 2).mean()
+        loss.backward()
+        opt.step()
+        return loss.detach()
+
+    first = step()
+    for _ in range(4):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = [step() for _ in range(200)]
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    # the same step captured once into a HIP graph and replayed (static tensors, Adam
+    # with its step count on the device; the library allocates nothing after its
+    # initialisation, the autograd buffers come from the graph's pool)
+    phg = torch.from_numpy(np.random.RandomState(0).uniform(0, 2 * np.pi, (n, n))
+                           .astype(np.float32)).to(dev).requires_grad_(True)
+    optg = torch.optim.Adam([phg], lr=0.1, capturable=True)
+
+    def body():
+        u1 = AG.propagate(u0_t, 1e-1, area, wl, phase=phg)
+        u2 = AG.propagate(u1, 5e-1, area, wl, lens_f=5e-1)
+        loss = ((u2.real ** 2 + u2.imag ** 2 - target) ** 2).mean()
+        loss.backward()
+        optg.step()
+        return loss
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(3):
+            optg.zero_grad(set_to_none=True)
+            body()
+    torch.cuda.current_stream().wait_stream(side)
+    graph = torch.cuda.CUDAGraph()
+    optg.zero_grad(set_to_none=True)
+    with torch.cuda.graph(graph):
+        static_loss = body()
+    for _ in range(2):
+        graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(200):
+        graph.replay()
+    torch.cuda.synchronize()
+    elg = time.perf_counter() - t0
+    return {"ms_per_step": round(el / 200 * 1e3, 4), "steps_per_s": round(200 / el, 1),
+            "ms_per_step_graph": round(elg / 200 * 1e3, 4),
+            "steps_per_s_graph": round(200 / elg, 1),
+            "loss_first": float(first), "loss_last": float(losses[-1]),
+            "loss_last_graph": float(static_loss),
+            "workload": "configuration 4: 512^2 phase mask, Gaussian 50 um, area 500 um, "
+                        "1064 nm, P(1 mm) -> lens 5 mm -> P(5 mm), MSE to a ring, Adam "
+                        "(lr 0.1), 200 timed steps after 5 warm-up steps; eager, and the "
+                        "step captured in one HIP graph"}
 
 
 def published_leg(torch, dev):
@@ -588,6 +670,7 @@ LEGS = {
     "speckle_chain": lambda torch, P, L, lib, dev: speckle_leg(torch, P, dev),
     "beam2d_chain": lambda torch, P, L, lib, dev: beam2d_leg(torch, P, dev),
     "published": lambda torch, P, L, lib, dev: published_leg(torch, dev),
+    "phase_mask_design": lambda torch, P, L, lib, dev: phase_mask_leg(torch, dev),
     "stack": lambda torch, P, L, lib, dev: stack_leg(torch, P, L, lib, dev),
     "large_fields": lambda torch, P, L, lib, dev: large_field_leg(torch, P, L, lib, dev),
     "resample": lambda torch, P, L, lib, dev: resample_leg(torch, P, dev),
