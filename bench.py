@@ -154,6 +154,7 @@ def extra_legs(torch, P, L, lib, dev):
            "beam2d_chain": beam2d_leg(torch, P, dev),
            "published": published_leg(torch, dev),
            "phase_mask_design": phase_mask_leg(torch, dev),
+           "gaussian_256_c128": gaussian_leg(torch, dev),
            "stack": stack_leg(torch, P, L, lib, dev)}
     out["large_fields"] = large_field_leg(torch, P, L, lib, dev)
     out["generators"] = generator_leg(torch, dev)
@@ -365,6 +366,34 @@ def stack_leg(torch, P, L, lib, dev):
                        "adjoints, and one checkpointed gradient step (loss sum(w |u_5|^2))")
     del u, scr, out, tmp, sr, w
     return res
+
+
+def gaussian_leg(torch, dev):
+    """Configuration 1 through the drop-in Beam2D on the device: 256^2 complex128 (the
+    reference's default precision), area 200 um, 632 nm, Gaussian beam of radius
+    25 um, one propagate(100 um), the intensity read on the device (the reference's
+    tests/gaussbeam_diffraction.py sequence).  Timed per whole sequence, construction
+    included, synchronised."""
+    from lightprop2d_amd import Beam2D, gaussian_beam, um, nm
+
+    def run():
+        b = Beam2D(200 * um, 256, 632 * nm, init_field_gen=gaussian_beam,
+                   init_gen_args=(1, 25 * um), numpy_output=False)
+        b.propagate(100 * um)
+        return b.iprofile
+
+    for _ in range(3):
+        run()
+    torch.cuda.synchronize()
+    reps = 100
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run()
+        torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / reps
+    return {"us_per_sequence": round(el * 1e6, 1), "sequences_per_s": round(1 / el, 1),
+            "workload": "configuration 1: Beam2D 256^2 complex128, Gaussian 25 um, "
+                        "propagate 100 um, iprofile on the device; construction included"}
 
 
 def phase_mask_leg(torch, dev):
@@ -671,6 +700,7 @@ LEGS = {
     "beam2d_chain": lambda torch, P, L, lib, dev: beam2d_leg(torch, P, dev),
     "published": lambda torch, P, L, lib, dev: published_leg(torch, dev),
     "phase_mask_design": lambda torch, P, L, lib, dev: phase_mask_leg(torch, dev),
+    "gaussian_256_c128": lambda torch, P, L, lib, dev: gaussian_leg(torch, dev),
     "stack": lambda torch, P, L, lib, dev: stack_leg(torch, P, L, lib, dev),
     "large_fields": lambda torch, P, L, lib, dev: large_field_leg(torch, P, L, lib, dev),
     "resample": lambda torch, P, L, lib, dev: resample_leg(torch, P, dev),
