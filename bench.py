@@ -351,7 +351,9 @@ def stack_leg(torch, P, L, lib, dev):
     def step():
         sr.grad = None
         v = AG.propagate_stack(ur, sr, dz, area, wl)
-        (w * (v.real ** 2 + v.imag ** 2)).sum().backward()
+        # |v|^2 through the real view: its backward is a view as well (no copies of the
+        # 1 GiB field into real and imaginary parts)
+        (w * torch.view_as_real(v).square().sum(-1)).sum().backward()
 
     step()
     torch.cuda.synchronize()
