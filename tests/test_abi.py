@@ -323,3 +323,27 @@ def test_phase_grad_argument_errors(lib):
     assert lib.lp2d_phase_grad(64, 1, 0, p, q, None, 0, None, 1, None) == _lib.LP2D_E_ARG
     assert lib.lp2d_phase_grad(64, 2, 0, p, q, p, 7, r, 1, None) == _lib.LP2D_E_ARG
     assert lib.lp2d_phase_grad(64, 1, 0, p, q, None, 0, p, 1, None) == _lib.LP2D_E_ARG
+
+
+def test_header_compiles_as_c_and_cpp(tmp_path):
+    """include/lp2d.h is a C header: a C99 translation unit (and a C++11 one) that
+    includes it and names a few of its types and entry points compiles cleanly with
+    warnings as errors."""
+    import shutil
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "use_lp2d.c"
+    src.write_text('#include "lp2d.h"\n'
+                   "int main(void) {\n"
+                   "  lp2d_plan* p = 0;\n"
+                   "  lp2d_optics o = {1.0, 1e-4, 0.1, 0.0, 0.0, 0, 0};\n"
+                   "  (void)o;\n"
+                   "  return (int)lp2d_plan_workspace_size(p, LP2D_OP_ADJOINT) + lp2d_version();\n"
+                   "}\n")
+    for cc, args in (("gcc", ["-std=c99", "-pedantic"]), ("g++", ["-std=c++11", "-x", "c++"])):
+        if shutil.which(cc) is None:
+            pytest.skip(f"{cc} not available")
+        r = subprocess.run([cc, *args, "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                            "-I" + os.path.join(root, "include"), str(src)],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
