@@ -282,27 +282,46 @@ __device__ __forceinline__ void line_fft(cx<T> (&v)[E], int t, const Acc& acc,
 // w^{3k} = w^{2k} w^k in registers (a rounding or two more than the table).  For
 // n = 4096 with 32 elements per thread this shrinks the tables from 7232 to 2112
 // entries, which lets a 4-column strip and its twiddles share the LDS.
+// CL = 2 (split compact last pass): w^k = w^{S*(k/S)} * w^{k%S} from two short tables
+// (S = 2^floor(log2(NS)/2) entries and NS/S entries, 64 instead of 1024 for n = 4096
+// with 32 elements per thread; one product more per butterfly), so that 2-row tiles
+// of 4096 points and their tables fit twice in one CU's LDS.
 constexpr int twl_radix(int N, int E, int ns) { return N / ns >= E ? E : N / ns; }
-constexpr bool twl_compact(int N, int E, int ns, bool CL) {
-  return CL && ns > 1 && ns * twl_radix(N, E, ns) == N && twl_radix(N, E, ns) <= 4;
+constexpr bool twl_compact(int N, int E, int ns, int CL) {
+  return CL != 0 && ns > 1 && ns * twl_radix(N, E, ns) == N && twl_radix(N, E, ns) <= 4;
 }
-constexpr int twl_off(int N, int E, int NS, bool CL = false) {
+constexpr int twl_split(int ns) {
+  int l = 0;
+  while ((2 << l) <= ns) ++l;
+  return 1 << (l / 2);
+}
+constexpr int twl_compact_len(int ns, int CL) {
+  return CL == 2 ? twl_split(ns) + ns / twl_split(ns) : ns;
+}
+constexpr int twl_off(int N, int E, int NS, int CL = 0) {
   int off = 0;
   for (int ns = 1; ns < NS; ns *= twl_radix(N, E, ns))
-    if (ns > 1) off += twl_compact(N, E, ns, CL) ? ns : ns * (twl_radix(N, E, ns) + 2);
+    if (ns > 1)
+      off += twl_compact(N, E, ns, CL) ? twl_compact_len(ns, CL)
+                                       : ns * (twl_radix(N, E, ns) + 2);
   return off;
 }
-template <int N, int E, bool CL = false>
+template <int N, int E, int CL = 0>
 struct TwlLen {
   static constexpr int value = twl_off(N, E, N, CL) > 0 ? twl_off(N, E, N, CL) : 1;
 };
 
 // Fill the per-pass tables from the concatenated global table (size L at offset L-2).
-template <typename T, int N, int E, int THREADS, int NS = 1, bool CL = false>
+template <typename T, int N, int E, int THREADS, int NS = 1, int CL = 0>
 __device__ __forceinline__ void twl_fill(cx<T>* twl, const cx<T>* __restrict__ tw, int tid) {
   if constexpr (NS < N) {
     constexpr int R = twl_radix(N, E, NS);
-    if constexpr (twl_compact(N, E, NS, CL)) {
+    if constexpr (twl_compact(N, E, NS, CL) && CL == 2) {
+      // w^i for i < S, then w^{S*h} for h < NS/S
+      constexpr int L = NS * R, OFF = twl_off(N, E, NS, CL), S = twl_split(NS);
+      for (int i = tid; i < S + NS / S; i += THREADS)
+        twl[OFF + i] = tw[(L - 2) + (i < S ? i : (i - S) * S)];
+    } else if constexpr (twl_compact(N, E, NS, CL)) {
       constexpr int L = NS * R, OFF = twl_off(N, E, NS, CL);
       for (int i = tid; i < NS; i += THREADS) twl[OFF + i] = tw[(L - 2) + i];
     } else if constexpr (NS > 1) {
@@ -318,7 +337,7 @@ __device__ __forceinline__ void twl_fill(cx<T>* twl, const cx<T>* __restrict__ t
 
 // The Stockham passes above with the twiddles read from the LDS tables.  WS: every
 // line's TL = N/E lanes lie in one wave, so the exchanges synchronise that wave only.
-template <typename T, int N, int E, bool INV, int NS, bool CL = false, bool WS = false,
+template <typename T, int N, int E, bool INV, int NS, int CL = 0, bool WS = false,
           typename Acc>
 __device__ __forceinline__ void stockham_l(cx<T> (&v)[E], int t, const Acc& acc,
                                            const cx<T>* twl) {
@@ -337,7 +356,12 @@ __device__ __forceinline__ void stockham_l(cx<T> (&v)[E], int t, const Acc& acc,
     if constexpr (twl_compact(N, E, NS, CL)) {
       // w^k from the table; w^{2k}, w^{3k} by products (R <= 4)
       cx<T> w[R > 1 ? R : 2];
-      w[1] = twl[twl_off(N, E, NS, CL) + k];
+      if constexpr (CL == 2) {
+        constexpr int S = twl_split(NS), OFF = twl_off(N, E, NS, CL);
+        w[1] = cmul(twl[OFF + S + k / S], twl[OFF + (k & (S - 1))]);
+      } else {
+        w[1] = twl[twl_off(N, E, NS, CL) + k];
+      }
       if constexpr (R > 2) w[2] = cmul(w[1], w[1]);
       if constexpr (R > 3) w[3] = cmul(w[2], w[1]);
 #pragma unroll
@@ -366,7 +390,7 @@ __device__ __forceinline__ void stockham_l(cx<T> (&v)[E], int t, const Acc& acc,
   }
 }
 
-template <typename T, int N, int E, bool INV, bool CL = false, bool WS = false, typename Acc>
+template <typename T, int N, int E, bool INV, int CL = 0, bool WS = false, typename Acc>
 __device__ __forceinline__ void line_fft_l(cx<T> (&v)[E], int t, const Acc& acc,
                                            const cx<T>* twl) {
   static_assert(!WS || 64 % (N / E) == 0, "a line's lanes within one wave");

@@ -89,7 +89,7 @@ void k_rows_chain(PassArgs<T> a) {
 
 // The same with the twiddles staged once in LDS (compact last pass when CL), each
 // workgroup taking every gridDim.x-th row tile of the batch (one resident round).
-template <typename T, int N, int E, int LINES, int PS, int WPE, bool CL>
+template <typename T, int N, int E, int LINES, int PS, int WPE, int CL>
 __global__ __launch_bounds__((N / E) * LINES) __attribute__((amdgpu_waves_per_eu(WPE)))
 void k_rows_chain_l(PassArgs<T> a, int nunits) {
   __shared__ cx<T> lds[LINES * RowPitch<N, PS, LINES>::value];
@@ -122,7 +122,7 @@ __global__ __launch_bounds__((N / E) * W) LP2D_OCC void k_cols(PassArgs<T> a, in
 // Pipelined column pass: grid (N/W strips) x (field groups of `per` <= kPipeMaxPer
 // fields); the twiddle tables of the strip's passes and the group's field pointers
 // and optics are staged in LDS once per workgroup.
-template <typename T, int N, int E, int W, int PS, int HOISTH = HOIST_NONE, bool CL = false>
+template <typename T, int N, int E, int W, int PS, int HOISTH = HOIST_NONE, int CL = 0>
 __global__ __launch_bounds__((N / E) * W) LP2D_OCC void k_cols_pipe(PassArgs<T> a, int per) {
   __shared__ cx<T> lds[W * PaddedLen<N, PS>::value];
   __shared__ cx<T> twl[TwlLen<N, E, CL>::value];
@@ -329,7 +329,7 @@ hipError_t launch_rows_chain(const PassArgs<T>& a, hipStream_t s) {
 
 // Adjoint chained pass: grid (N/LINES row tiles) x (field groups); twiddles in LDS
 // (TWL) or from the global tables.
-template <typename T, int N, int E, int LINES, int PS, bool CL, bool TWL, bool LAST>
+template <typename T, int N, int E, int LINES, int PS, int CL, bool TWL, bool LAST>
 __global__ __launch_bounds__((N / E) * LINES) LP2D_OCC
 void k_rows_adj_chain(PassArgs<T> a, int groups, float* partial) {
   __shared__ cx<T> lds[LINES * RowPitch<N, PS, LINES>::value];
@@ -347,12 +347,12 @@ void k_rows_adj_chain(PassArgs<T> a, int groups, float* partial) {
       TWL ? twl : nullptr, threadIdx.x);
 }
 
-template <typename T, int N, int E, int LINES, bool CL>
+template <typename T, int N, int E, int LINES, int CL>
 constexpr int chain_l_lds() {
   return (LINES * RowPitch<N, cmax(1, ilog2c(E)), LINES>::value + TwlLen<N, E, CL>::value) *
          (int)sizeof(cx<T>);
 }
-template <typename T, int N, int E, int LINES, int PS, int WPE, bool CL>
+template <typename T, int N, int E, int LINES, int PS, int WPE, int CL>
 hipError_t launch_rows_chain_l(const PassArgs<T>& a, hipStream_t s) {
   if constexpr (chain_ok<T, N, E, LINES, PS>() && chain_l_lds<T, N, E, LINES, CL>() <= kLdsBudget) {
     if (!(a.flags & F_TILED) || !a.phase || a.src_tab || a.dst_tab) return hipErrorInvalidValue;
@@ -372,13 +372,13 @@ hipError_t launch_rows_chain_l(const PassArgs<T>& a, hipStream_t s) {
   }
 }
 
-template <typename T, int N, int E, int LINES, bool CL, bool TWL>
+template <typename T, int N, int E, int LINES, int CL, bool TWL>
 constexpr int adj_chain_lds() {
   return (LINES * RowPitch<N, cmax(1, ilog2c(E)), LINES>::value +
           (TWL ? TwlLen<N, E, CL>::value : 0)) *
          (int)sizeof(cx<T>);
 }
-template <typename T, int N, int E, int LINES, int PS, bool CL, bool TWL, bool LAST>
+template <typename T, int N, int E, int LINES, int PS, int CL, bool TWL, bool LAST>
 hipError_t launch_rows_adj_chain(const PassArgs<T>& a, int groups, float* partial, hipStream_t s) {
   if constexpr (chain_ok<T, N, E, LINES, PS>() &&
                 adj_chain_lds<T, N, E, LINES, CL, TWL>() <= kLdsBudget) {
@@ -463,7 +463,7 @@ hipError_t launch_cols(const PassArgs<T>& a, int mode, hipStream_t s) {
 // multipliers, other evanescent modes, single-direction passes) runs the plain class
 // of the same shape.  Each workgroup takes `per` fields so that the grid is about one
 // round of resident workgroups.
-template <typename T, int N, int E, int W, int PS, int HOISTH = HOIST_NONE, bool CL = false>
+template <typename T, int N, int E, int W, int PS, int HOISTH = HOIST_NONE, int CL = 0>
 hipError_t launch_cols_pipe(const PassArgs<T>& a, int mode, hipStream_t s) {
   const bool ok = pipe_eligible(a, mode);
   // the block-tiled layout is only defined for strips of tile_cols(N) columns, and only
@@ -582,7 +582,7 @@ void add_col(Registry<T>& r) {
                               S::LDS, HOIST ? 1 : 0};
 }
 
-template <typename T, int N, int E, int WT, int HOISTH = HOIST_NONE, bool CL = false>
+template <typename T, int N, int E, int WT, int HOISTH = HOIST_NONE, int CL = 0>
 void add_col_pipe(Registry<T>& r) {
   using S = ColShape<T, N, E, WT>;
   constexpr int lg = ilog2c(N);
@@ -637,8 +637,9 @@ void register_n(Registry<T>& r) {
   // per SIMD) with global or LDS-staged twiddles, and 16 elements per thread in row
   // tiles of at least one tile height (4 waves per SIMD).  Variant 0 is the fastest
   // measured per size (stack of 5 screens, per pass; 32 global / 32 LDS / 16):
-  // 4096 x8: 686 / 627 / 846 us; 1024 x32: 142 / 117 / 148; 512 x64: 67 / 63 / 81;
-  // 256 x128: 36 / 34 / 33.
+  // 4096 x8: 680 / 627 / 846 us; 1024 x32: 142 / 117 / 148; 512 x64: 67 / 63 / 81;
+  // 256 x128: 36 / 34 / 33.  At 4096 the 2-row tiles with the split compact table
+  // (75 KiB of LDS, two workgroups per CU) take 529 us and come first.
   r.nchain[lg] = 0;
   for (int k = 0; k < 4; ++k) r.chain_twl[lg][k] = 0;
   r.adj_chain[lg][0] = r.adj_chain[lg][1] = nullptr;
@@ -668,7 +669,7 @@ void register_n(Registry<T>& r) {
     // twiddles in LDS: the 32-element shape (rows of the row pass, or 4 rows at 4096
     // with the compact last-pass table: one 512-thread workgroup per CU)
     constexpr int LL = N >= 4096 ? 4 : S::LINES;
-    constexpr bool CLL = N >= 4096;
+    constexpr int CLL = N >= 4096 ? 1 : 0;
     constexpr bool okl = chain_ok<T, N, 32, LL, S::PS>() &&
                          chain_l_lds<T, N, 32, LL, CLL>() <= kLdsBudget &&
                          (N / 32) * LL <= 1024;
@@ -681,7 +682,24 @@ void register_n(Registry<T>& r) {
         r.chain_twl[lg][k] = 1;
       }
     };
-    if constexpr (N >= 512) {
+    // 2-row tiles of 4096 points with the split compact table: two workgroups per CU
+    constexpr bool oks = N == 4096 && chain_ok<T, N, 32, S::LINES, S::PS>() &&
+                         chain_l_lds<T, N, 32, S::LINES, 2>() <= 80 * 1024;
+    auto add32s = [&] {
+      if constexpr (oks) {
+        const int k = r.nchain[lg]++;
+        r.chain[lg][k] = &launch_rows_chain_l<T, N, 32, S::LINES, S::PS, 2, 2>;
+        r.chain_E[lg][k] = 32;
+        r.chain_lines[lg][k] = S::LINES;
+        r.chain_twl[lg][k] = 1;
+      }
+    };
+    if constexpr (N == 4096) {
+      add32s();
+      add32l();
+      add32();
+      add16();
+    } else if constexpr (N >= 512) {
       add32l();
       add32();
       add16();

@@ -657,7 +657,7 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
 // lines.  Three sweeps per plane become two.
 // twl: null for the twiddles of the global table; otherwise the per-pass LDS tables of
 // twl_fill (compact last pass when CL).
-template <typename T, int N, int E, int LINES, int PS, bool CL = false>
+template <typename T, int N, int E, int LINES, int PS, int CL = 0>
 __device__ __forceinline__ void rows_chain_body(const PassArgs<T>& a, int b, int y0,
                                                 cx<T>* lds, const cx<T>* twl, int tid) {
   constexpr int TL = N / E;
@@ -709,7 +709,7 @@ __device__ __forceinline__ void rows_chain_body(const PassArgs<T>& a, int b, int
 // adjoint.  The group's sums go to `partial` (n x n) once; with partial null no
 // gradient is formed (u is not read).  Twiddles from the LDS tables twl, or from the
 // global tables when twl is null.
-template <typename T, int N, int E, int LINES, int PS, bool CL, bool LAST>
+template <typename T, int N, int E, int LINES, int PS, int CL, bool LAST>
 __device__ __forceinline__ void rows_adj_chain_body(const PassArgs<T>& a, int b0, int b1, int y0,
                                                     float* __restrict__ partial, cx<T>* lds,
                                                     const cx<T>* twl, int tid) {
@@ -876,7 +876,7 @@ __device__ __forceinline__ void rows_pipe_body(const PassArgs<T>& a, int u0, int
       }
     }
     if (u + ustep < nunits) fetch(u + ustep);  // uniform branch
-    line_fft_l<T, N, E, INV, false, WS>(cur, t, acc, twl);
+    line_fft_l<T, N, E, INV, 0, WS>(cur, t, acc, twl);
     if (scale) {
 #pragma unroll
       for (int m = 0; m < E; ++m) cur[m] = cscale(cur[m], a.scale);
@@ -1235,7 +1235,7 @@ inline bool pipe_eligible(const PassArgs<T>& a, int mode) {
 // 118.9 us, factor 113.9, phase 121.8; 256^2 x64 with 16 elements 22.6 us, phase 20.2,
 // factor 22.2; 512^2 x64 with 16: 66.3, phase 62.0, factor 62.7.
 enum : int { HOIST_NONE = 0, HOIST_FACTOR = 1, HOIST_PHASE = 2 };
-template <typename T, int N, int E, int W, int PS, int HOISTH = HOIST_NONE, bool CL = false>
+template <typename T, int N, int E, int W, int PS, int HOISTH = HOIST_NONE, int CL = 0>
 __device__ __forceinline__ void cols_pipe_body(const PassArgs<T>& a, int b0, int b1, int x0,
                                                cx<T>* lds, const cx<T>* twl,
                                                const PipeField* pf, int tid) {
@@ -1456,7 +1456,7 @@ __device__ __forceinline__ void small_pipe_body(const PassArgs<T>& a, int b0, in
     if (b + 1 < b1) fetch(b + 1);  // uniform branch
     {  // rows: forward FFT into the tile
       LineAcc<T, PS> acc{lds + y * P};
-      line_fft_l<T, N, E, false, false, true>(cur, tr, acc, twl);
+      line_fft_l<T, N, E, false, 0, true>(cur, tr, acc, twl);
 #pragma unroll
       for (int m = 0; m < E; ++m) acc.st(tr + TL * m, cur[m]);
     }
@@ -1467,7 +1467,7 @@ __device__ __forceinline__ void small_pipe_body(const PassArgs<T>& a, int b0, in
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = acc.ld(tc + TL * m);
       wave_sync();  // the column's own lanes read it before its transform rewrites it
-      line_fft_l<T, N, E, false, false, true>(v, tc, acc, twl);
+      line_fft_l<T, N, E, false, 0, true>(v, tc, acc, twl);
       if (mid) {
         ColMid cm = col_mid(pf[b - b0].o, flags, sx);
         int tt = tc;
@@ -1480,7 +1480,7 @@ __device__ __forceinline__ void small_pipe_body(const PassArgs<T>& a, int b0, in
 #pragma unroll
         for (int m = 0; m < E; ++m) gst<NT_SMALL>(so + ((long long)(tc + TL * m) * N), v[m]);
       }
-      line_fft_l<T, N, E, true, false, true>(v, tc, acc, twl);
+      line_fft_l<T, N, E, true, 0, true>(v, tc, acc, twl);
       wave_sync();
 #pragma unroll
       for (int m = 0; m < E; ++m) acc.st(tc + TL * m, v[m]);
@@ -1492,7 +1492,7 @@ __device__ __forceinline__ void small_pipe_body(const PassArgs<T>& a, int b0, in
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = acc.ld(tr + TL * m);
       wave_sync();  // the row's own lanes read it before its transform rewrites it
-      line_fft_l<T, N, E, true, false, true>(v, tr, acc, twl);
+      line_fft_l<T, N, E, true, 0, true>(v, tr, acc, twl);
       if (scale) {
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = cscale(v[m], a.scale);
