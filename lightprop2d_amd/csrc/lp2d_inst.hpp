@@ -625,8 +625,11 @@ void register_n(Registry<T>& r) {
       add_col_pipe<T, N, 32, 8, HOIST_FACTOR>(r);
       add_col_pipe<T, N, 16, 8, HOIST_PHASE>(r);
       // the compact twiddle table lets 4-column strips of 4096 points pipeline (the
-      // hoisted factor does not fit in the registers beside them)
+      // hoisted factor does not fit in the registers beside them); at 2048 the split
+      // compact table brings 4-column strips to 77 KiB of LDS, two workgroups per CU
+      // (163 against 223 us per 8 fields)
       if constexpr (N >= 4096) add_col_pipe<T, N, 32, 4, HOIST_NONE, true>(r);
+      if constexpr (N == 2048) add_col_pipe<T, N, 32, 4, HOIST_NONE, 2>(r);
       add_row_pipe<T, N, 32>(r);
     } else {
       add_row<T, N, 16>(r);
