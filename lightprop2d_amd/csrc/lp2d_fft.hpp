@@ -284,11 +284,14 @@ __device__ __forceinline__ void line_fft(cx<T> (&v)[E], int t, const Acc& acc,
 // entries, which lets a 4-column strip and its twiddles share the LDS.
 // CL = 2 (split compact last pass): w^k = w^{S*(k/S)} * w^{k%S} from two short tables
 // (S = 2^floor(log2(NS)/2) entries and NS/S entries, 64 instead of 1024 for n = 4096
-// with 32 elements per thread; one product more per butterfly), so that 2-row tiles
-// of 4096 points and their tables fit twice in one CU's LDS.
+// with 32 elements per thread; one product more per butterfly), and last-pass radices
+// up to 16, whose w^{rk} come from the product tree w^{rk} = w^{(r-r/2)k} w^{(r/2)k}
+// (at most 1 + log2(r) roundings); so that 2-row tiles of 4096 points and their
+// tables fit twice in one CU's LDS.
 constexpr int twl_radix(int N, int E, int ns) { return N / ns >= E ? E : N / ns; }
 constexpr bool twl_compact(int N, int E, int ns, int CL) {
-  return CL != 0 && ns > 1 && ns * twl_radix(N, E, ns) == N && twl_radix(N, E, ns) <= 4;
+  return CL != 0 && ns > 1 && ns * twl_radix(N, E, ns) == N &&
+         twl_radix(N, E, ns) <= (CL == 2 ? 16 : 4);
 }
 constexpr int twl_split(int ns) {
   int l = 0;
@@ -362,8 +365,8 @@ __device__ __forceinline__ void stockham_l(cx<T> (&v)[E], int t, const Acc& acc,
       } else {
         w[1] = twl[twl_off(N, E, NS, CL) + k];
       }
-      if constexpr (R > 2) w[2] = cmul(w[1], w[1]);
-      if constexpr (R > 3) w[3] = cmul(w[2], w[1]);
+#pragma unroll
+      for (int r = 2; r < R; ++r) w[r] = cmul(w[r - r / 2], w[r / 2]);
 #pragma unroll
       for (int r = 1; r < R; ++r) a[r] = INV ? cmulc(a[r], w[r]) : cmul(a[r], w[r]);
     } else if constexpr (NS > 1) {

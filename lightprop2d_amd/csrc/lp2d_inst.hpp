@@ -329,8 +329,8 @@ hipError_t launch_rows_chain(const PassArgs<T>& a, hipStream_t s) {
 
 // Adjoint chained pass: grid (N/LINES row tiles) x (field groups); twiddles in LDS
 // (TWL) or from the global tables.
-template <typename T, int N, int E, int LINES, int PS, int CL, bool TWL, bool LAST>
-__global__ __launch_bounds__((N / E) * LINES) LP2D_OCC
+template <typename T, int N, int E, int LINES, int PS, int CL, bool TWL, bool LAST, int WPE = 2>
+__global__ __launch_bounds__((N / E) * LINES) __attribute__((amdgpu_waves_per_eu(WPE)))
 void k_rows_adj_chain(PassArgs<T> a, int groups, float* partial) {
   __shared__ cx<T> lds[LINES * RowPitch<N, PS, LINES>::value];
   __shared__ cx<T> twl[TWL ? TwlLen<N, E, CL>::value : 1];
@@ -378,7 +378,7 @@ constexpr int adj_chain_lds() {
           (TWL ? TwlLen<N, E, CL>::value : 0)) *
          (int)sizeof(cx<T>);
 }
-template <typename T, int N, int E, int LINES, int PS, int CL, bool TWL, bool LAST>
+template <typename T, int N, int E, int LINES, int PS, int CL, bool TWL, bool LAST, int WPE = 2>
 hipError_t launch_rows_adj_chain(const PassArgs<T>& a, int groups, float* partial, hipStream_t s) {
   if constexpr (chain_ok<T, N, E, LINES, PS>() &&
                 adj_chain_lds<T, N, E, LINES, CL, TWL>() <= kLdsBudget) {
@@ -387,7 +387,7 @@ hipError_t launch_rows_adj_chain(const PassArgs<T>& a, int groups, float* partia
       return hipErrorInvalidValue;
     constexpr int threads = (N / E) * LINES;
     const long long grid = (long long)(N / LINES) * groups;
-    k_rows_adj_chain<T, N, E, LINES, PS, CL, TWL, LAST><<<(unsigned)grid, threads, 0, s>>>(
+    k_rows_adj_chain<T, N, E, LINES, PS, CL, TWL, LAST, WPE><<<(unsigned)grid, threads, 0, s>>>(
         a, groups, partial);
     return hipGetLastError();
   } else {
@@ -716,13 +716,17 @@ void register_n(Registry<T>& r) {
     // the chained form spilled 280-400 bytes per lane, and so does a 1024-thread
     // workgroup held to 128 registers, so 2048 and 8192 keep the plane-by-plane
     // adjoint), compact last-pass twiddles in LDS where two workgroups per CU keep
-    // room for them, the global tables otherwise
+    // room for them (at 4096 the split table with the radix-16 last pass, and the
+    // passes held to 128 registers for four waves per SIMD: 957 -> 617 us and 689 ->
+    // 519 us per 8 fields against the global tables), the global tables otherwise
     constexpr int LA = L16;
-    constexpr bool TWA = adj_chain_lds<T, N, 16, LA, true, true>() <= 80 * 1024;
+    constexpr int CLA = N == 4096 ? 2 : 1;
+    constexpr bool TWA = adj_chain_lds<T, N, 16, LA, CLA, true>() <= 80 * 1024;
     if constexpr (ok16 && (N / 16) * LA <= 512 &&
-                  adj_chain_lds<T, N, 16, LA, true, TWA>() <= kLdsBudget) {
-      r.adj_chain[lg][0] = &launch_rows_adj_chain<T, N, 16, LA, PS16, true, TWA, false>;
-      r.adj_chain[lg][1] = &launch_rows_adj_chain<T, N, 16, LA, PS16, true, TWA, true>;
+                  adj_chain_lds<T, N, 16, LA, CLA, TWA>() <= kLdsBudget) {
+      constexpr int WPA = N == 4096 ? 4 : 2;
+      r.adj_chain[lg][0] = &launch_rows_adj_chain<T, N, 16, LA, PS16, CLA, TWA, false, WPA>;
+      r.adj_chain[lg][1] = &launch_rows_adj_chain<T, N, 16, LA, PS16, CLA, TWA, true, WPA>;
       r.adj_threads[lg] = (N / 16) * LA;
       r.adj_lines[lg] = LA;
     }
