@@ -1673,7 +1673,7 @@ int lp2d_variant_name(int n, int dtype, int kind, int variant, char* buf, int bu
     const int lds = dtype ? g_rd->rows[lg][variant].lds : g_rf->rows[lg][variant].lds;
     const int rk = dtype ? g_rd->rows[lg][variant].kind : g_rf->rows[lg][variant].kind;
     return snprintf(buf, buflen, "rows E=%d lines=%d threads=%d lds=%d%s", E, L, th, lds,
-                    (rk & 2) ? " pipelined" : "");
+                    (rk & 2) ? " pipelined" : (rk & 4) ? " lds-twiddles" : "");
   }
   const int E = dtype ? g_rd->cols[lg][variant].E : g_rf->cols[lg][variant].E;
   const int W = dtype ? g_rd->cols[lg][variant].W : g_rf->cols[lg][variant].W;

@@ -29,6 +29,22 @@ __global__ __launch_bounds__((N / E) * LINES) LP2D_OCC void k_rows(PassArgs<T> a
   rows_body<T, N, E, LINES, INV, PS, FEAT>(a, b, y0, cmin(LINES, N - y0), lds, threadIdx.x);
 }
 
+// The row pass with its twiddles staged once in LDS (compact mode CL), each workgroup
+// taking every gridDim.x-th row tile of the batch (one resident round).
+template <typename T, int N, int E, int LINES, bool INV, int PS, bool FEAT, int CL>
+__global__ __launch_bounds__((N / E) * LINES) LP2D_OCC void k_rows_l(PassArgs<T> a, int nunits) {
+  __shared__ cx<T> lds[LINES * RowPitch<N, PS, LINES>::value];
+  __shared__ cx<T> twl[TwlLen<N, E, CL>::value];
+  twl_fill<T, N, E, (N / E) * LINES, 1, CL>(twl, a.tw, threadIdx.x);
+  __syncthreads();
+  constexpr int TPF = N / LINES;
+  for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const int b = u / TPF;
+    rows_body<T, N, E, LINES, INV, PS, FEAT, CL>(a, b, (u - b * TPF) * LINES, LINES, lds,
+                                                 threadIdx.x, twl);
+  }
+}
+
 // Pipelined row pass: about one resident round of workgroups, each taking every
 // gridDim.x-th row tile of the batch; the twiddle tables are staged in LDS once.
 template <typename T, int N, int E, int LINES, bool INV, int PS, bool TILED, bool PH = false,
@@ -226,6 +242,31 @@ hipError_t launch_rows(const PassArgs<T>& a, bool inv, hipStream_t s) {
       k_rows<T, N, E, LINES, false, PS, false><<<grid, threads, 0, s>>>(a);
   }
   return hipGetLastError();
+}
+
+template <typename T, int N, int E, int LINES, bool INV, int PS, bool FEAT, int CL>
+hipError_t run_rows_l(const PassArgs<T>& a, int nu, hipStream_t s) {
+  constexpr int threads = (N / E) * LINES;
+  static std::atomic<int> slot{0};
+  const int sl = cached_slots(slot, (const void*)&k_rows_l<T, N, E, LINES, INV, PS, FEAT, CL>,
+                              threads);
+  const int grid = nu < sl ? nu : sl;
+  k_rows_l<T, N, E, LINES, INV, PS, FEAT, CL><<<grid, threads, 0, s>>>(a, nu);
+  return hipGetLastError();
+}
+template <typename T, int N, int E, int LINES, int PS, int CL>
+hipError_t launch_rows_l(const PassArgs<T>& a, bool inv, hipStream_t s) {
+  if ((a.flags & F_TILED) && (LINES % tile_rows(N) != 0 || (N / E) % tile_cols(N) != 0))
+    return hipErrorInvalidValue;
+  // passes with spatial features keep the plain class (in the persistent loop their
+  // extra state spills: 636-748 bytes per lane at 4096 points)
+  const long long units = (long long)a.batch * (N / LINES);
+  const bool feat = (a.flags & (F_SPATIAL | F_GRAD | F_INTENSITY | F_NO_STORE)) != 0;
+  if (feat || N % LINES != 0 || units > 0x7fffffff)
+    return launch_rows<T, N, E, LINES, PS>(a, inv, s);
+  const int nu = (int)units;
+  return inv ? run_rows_l<T, N, E, LINES, true, PS, false, CL>(a, nu, s)
+             : run_rows_l<T, N, E, LINES, false, PS, false, CL>(a, nu, s);
 }
 
 // The pipelined class covers the row passes of strided complex64 batches without
@@ -572,6 +613,20 @@ void add_row_pipe(Registry<T>& r) {
                                 S::LINES, S::THREADS, lds, S::LINES, 2};
 }
 
+// The plain row pass with LDS twiddles (kind bit 2), compact mode CL.
+template <typename T, int N, int E, int CL>
+void add_row_l(Registry<T>& r) {
+  using S = RowShape<T, N, E>;
+  constexpr int lg = ilog2c(N);
+  int& k = r.nrows[lg];
+  if (k >= kMaxVariants) return;
+  constexpr int lds = S::LDS + TwlLen<N, E, CL>::value * (int)sizeof(cx<T>);
+  if constexpr (lds <= kLdsBudget)
+    r.rows[lg][k++] = RowVar<T>{&launch_rows_l<T, N, E, S::LINES, S::PS, CL>,
+                                &launch_rows_grad<T, N, E, S::LINES, S::PS, (E <= 8)>, E,
+                                S::LINES, S::THREADS, lds, S::LINES, 4};
+}
+
 template <typename T, int N, int E, int WT, bool HOIST = false>
 void add_col(Registry<T>& r) {
   using S = ColShape<T, N, E, WT>;
@@ -631,6 +686,7 @@ void register_n(Registry<T>& r) {
       if constexpr (N >= 4096) add_col_pipe<T, N, 32, 4, HOIST_NONE, true>(r);
       if constexpr (N == 2048) add_col_pipe<T, N, 32, 4, HOIST_NONE, 2>(r);
       add_row_pipe<T, N, 32>(r);
+      if constexpr (N == 4096) add_row_l<T, N, 32, 2>(r);
     } else {
       add_row<T, N, 16>(r);
       add_col<T, N, 16, 8>(r);

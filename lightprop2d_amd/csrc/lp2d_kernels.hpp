@@ -561,9 +561,11 @@ struct RowPitch {
   static constexpr int value = PaddedLen<N, PS>::value + (LINES >= 8 ? 4 : 0);
 };
 
-template <typename T, int N, int E, int LINES, bool INV, int PS, bool FEAT = true>
+// TW < 0: twiddles from the global tables; otherwise from the LDS tables twl of
+// twl_fill with compact mode TW.
+template <typename T, int N, int E, int LINES, bool INV, int PS, bool FEAT = true, int TW = -1>
 __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, int nrows,
-                                          cx<T>* lds, int tid) {
+                                          cx<T>* lds, int tid, const cx<T>* twl = nullptr) {
   constexpr int TL = N / E;
   constexpr int L = RowPitch<N, PS, LINES>::value;
   constexpr int TR = tile_rows(N), TC = tile_cols(N);
@@ -606,7 +608,10 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
     }
   }
   LineAcc<T, PS> acc{lds + l * L};
-  line_fft<T, N, E, INV>(v, t, acc, (INV ? a.tw_inv : a.tw) + (N - 2));
+  if constexpr (TW >= 0)
+    line_fft_l<T, N, E, INV, TW>(v, t, acc, twl);
+  else
+    line_fft<T, N, E, INV>(v, t, acc, (INV ? a.tw_inv : a.tw) + (N - 2));
   LP2D_GLB cx<T>* __restrict__ dst = field_dst(a, b) + ubase;
   // whole lines per store instruction: natural map / tiled-plane map
   constexpr bool WN = nt_store_site<T>(1, TL) != 0;

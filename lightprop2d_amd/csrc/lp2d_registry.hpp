@@ -31,6 +31,7 @@ struct RowVar {
   int E, lines, threads, lds;
   int grad_lines;  // row tile of grad_fn: whole 8-row blocks where the threads allow
   int kind;        // bit 1: software-pipelined over the row tiles of the batch
+                   // bit 2: twiddles staged in LDS, one resident round of workgroups
 };
 template <typename T>
 struct ColVar {
