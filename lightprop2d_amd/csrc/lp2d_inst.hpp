@@ -31,7 +31,8 @@ __global__ __launch_bounds__((N / E) * LINES) LP2D_OCC void k_rows(PassArgs<T> a
 
 // The row pass with its twiddles staged once in LDS (compact mode CL), each workgroup
 // taking every gridDim.x-th row tile of the batch (one resident round).
-template <typename T, int N, int E, int LINES, bool INV, int PS, bool FEAT, int CL>
+template <typename T, int N, int E, int LINES, bool INV, int PS, bool FEAT, int CL,
+          bool PHO = false>
 __global__ __launch_bounds__((N / E) * LINES) LP2D_OCC void k_rows_l(PassArgs<T> a, int nunits) {
   __shared__ cx<T> lds[LINES * RowPitch<N, PS, LINES>::value];
   __shared__ cx<T> twl[TwlLen<N, E, CL>::value];
@@ -40,8 +41,8 @@ __global__ __launch_bounds__((N / E) * LINES) LP2D_OCC void k_rows_l(PassArgs<T>
   constexpr int TPF = N / LINES;
   for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
     const int b = u / TPF;
-    rows_body<T, N, E, LINES, INV, PS, FEAT, CL>(a, b, (u - b * TPF) * LINES, LINES, lds,
-                                                 threadIdx.x, twl);
+    rows_body<T, N, E, LINES, INV, PS, FEAT, CL, PHO>(a, b, (u - b * TPF) * LINES, LINES, lds,
+                                                      threadIdx.x, twl);
   }
 }
 
@@ -244,14 +245,14 @@ hipError_t launch_rows(const PassArgs<T>& a, bool inv, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <typename T, int N, int E, int LINES, bool INV, int PS, bool FEAT, int CL>
+template <typename T, int N, int E, int LINES, bool INV, int PS, bool FEAT, int CL, bool PHO>
 hipError_t run_rows_l(const PassArgs<T>& a, int nu, hipStream_t s) {
   constexpr int threads = (N / E) * LINES;
   static std::atomic<int> slot{0};
-  const int sl = cached_slots(slot, (const void*)&k_rows_l<T, N, E, LINES, INV, PS, FEAT, CL>,
-                              threads);
+  const int sl = cached_slots(
+      slot, (const void*)&k_rows_l<T, N, E, LINES, INV, PS, FEAT, CL, PHO>, threads);
   const int grid = nu < sl ? nu : sl;
-  k_rows_l<T, N, E, LINES, INV, PS, FEAT, CL><<<grid, threads, 0, s>>>(a, nu);
+  k_rows_l<T, N, E, LINES, INV, PS, FEAT, CL, PHO><<<grid, threads, 0, s>>>(a, nu);
   return hipGetLastError();
 }
 template <typename T, int N, int E, int LINES, int PS, int CL>
@@ -259,14 +260,17 @@ hipError_t launch_rows_l(const PassArgs<T>& a, bool inv, hipStream_t s) {
   if ((a.flags & F_TILED) && (LINES % tile_rows(N) != 0 || (N / E) % tile_cols(N) != 0))
     return hipErrorInvalidValue;
   // passes with spatial features keep the plain class (in the persistent loop their
-  // extra state spills: 636-748 bytes per lane at 4096 points)
+  // extra state spills: 636-748 bytes per lane at 4096 points), except a forward pass
+  // whose only feature is a phase screen
   const long long units = (long long)a.batch * (N / LINES);
-  const bool feat = (a.flags & (F_SPATIAL | F_GRAD | F_INTENSITY | F_NO_STORE)) != 0;
-  if (feat || N % LINES != 0 || units > 0x7fffffff)
+  const int fl = a.flags & (F_SPATIAL | F_GRAD | F_INTENSITY | F_NO_STORE);
+  const bool pho = !inv && fl == F_PHASE && a.phase;
+  if ((fl != 0 && !pho) || N % LINES != 0 || units > 0x7fffffff)
     return launch_rows<T, N, E, LINES, PS>(a, inv, s);
   const int nu = (int)units;
-  return inv ? run_rows_l<T, N, E, LINES, true, PS, false, CL>(a, nu, s)
-             : run_rows_l<T, N, E, LINES, false, PS, false, CL>(a, nu, s);
+  if (pho) return run_rows_l<T, N, E, LINES, false, PS, false, CL, true>(a, nu, s);
+  return inv ? run_rows_l<T, N, E, LINES, true, PS, false, CL, false>(a, nu, s)
+             : run_rows_l<T, N, E, LINES, false, PS, false, CL, false>(a, nu, s);
 }
 
 // The pipelined class covers the row passes of strided complex64 batches without

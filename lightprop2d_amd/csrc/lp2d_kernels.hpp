@@ -562,8 +562,10 @@ struct RowPitch {
 };
 
 // TW < 0: twiddles from the global tables; otherwise from the LDS tables twl of
-// twl_fill with compact mode TW.
-template <typename T, int N, int E, int LINES, bool INV, int PS, bool FEAT = true, int TW = -1>
+// twl_fill with compact mode TW.  PHO (with FEAT off): the phase screen alone, applied
+// before the transform as the featured prologue applies it.
+template <typename T, int N, int E, int LINES, bool INV, int PS, bool FEAT = true, int TW = -1,
+          bool PHO = false>
 __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, int nrows,
                                           cx<T>* lds, int tid, const cx<T>* twl = nullptr) {
   constexpr int TL = N / E;
@@ -597,6 +599,7 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = gld<nt_row_site<T, TL>()>(src + (lo + (unsigned)(TL * m)));
   }
+  if constexpr (!FEAT && PHO) phase_mul_all<T, E, false>(a, v, b, y, t, TL, N);
   if constexpr (FEAT) {
     if (a.flags & F_SPATIAL) {
       spatial_pro_all<T, E>(a, v, b, y, t, TL, N);
