@@ -201,9 +201,11 @@ enum : int {
   NT_GRAD = 16,                    // adjoint gradient row pass
   NT_SMALL = 32,                   // whole-field kernels
   NT_ROWI_ST = 64,                 // row passes: stores of the inverse pass
+  NT_TILE_ST = 128,                // whole-tile stores into the block-tiled plane
+                                   // (4096 points: LDS-twiddle rows, chained passes)
 };
 #ifndef LP2D_NT
-#define LP2D_NT (NT_ROW_LD | NT_GRAD | NT_ROWI_ST)
+#define LP2D_NT (NT_ROW_LD | NT_GRAD | NT_ROWI_ST | NT_TILE_ST)
 #endif
 template <int SITE, typename T>
 __device__ __forceinline__ cx<T> gld(const LP2D_GLB cx<T>* p) {
@@ -562,7 +564,10 @@ struct RowPitch {
 };
 
 // TW < 0: twiddles from the global tables; otherwise from the LDS tables twl of
-// twl_fill with compact mode TW.  PHO (with FEAT off): the phase screen alone, applied
+// twl_fill with compact mode TW, and a forward pass into the block-tiled plane takes
+// the whole-tile thread map with nontemporal stores (every store instruction writes
+// whole lines; the column pass then reads the plane faster: 786 -> 739 us at 8 x
+// 4096^2 for 22 us more here).  PHO (with FEAT off): the phase screen alone, applied
 // before the transform as the featured prologue applies it.
 template <typename T, int N, int E, int LINES, bool INV, int PS, bool FEAT = true, int TW = -1,
           bool PHO = false>
@@ -577,7 +582,7 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
   // so every load instruction of the wave reads whole tiles (8 x 8: one 512-byte tile).
   // (The same map for the forward pass of 2 x 4 tiles measured no different.)
   int l = tid / TL, t = tid - (tid / TL) * TL;
-  if constexpr (INV && TiledRowMap<N, TL, LINES>::ok) {
+  if constexpr ((INV || TW >= 0) && TiledRowMap<N, TL, LINES>::ok) {
     if (a.flags & F_TILED) TiledRowMap<N, TL, LINES>::map(tid, l, t);
   }
   const int y = valid_row(y0 + l, y0 + nrows - 1);
@@ -642,7 +647,7 @@ __device__ __forceinline__ void rows_body(const PassArgs<T>& a, int b, int y0, i
     if (!INV && tiled) {
 #pragma unroll
       for (int m = 0; m < E; ++m)
-        gst<INV ? NT_ROWI_ST : NT_ROW_ST>(dst + (lo_t + (unsigned)(tile_rows(N) * TL * m)), mk(v[m].re * sc, v[m].im * sc));
+        gst<TW >= 0 ? NT_TILE_ST : NT_ROW_ST>(dst + (lo_t + (unsigned)(tile_rows(N) * TL * m)), mk(v[m].re * sc, v[m].im * sc));
     } else if (tiled ? WT : WN) {
 #pragma unroll
       for (int m = 0; m < E; ++m)
@@ -703,8 +708,10 @@ __device__ __forceinline__ void rows_chain_body(const PassArgs<T>& a, int b, int
   else
     line_fft<T, N, E, false>(v, t, acc, a.tw + (N - 2));
   LP2D_GLB cx<T>* __restrict__ dst = field_dst(a, b) + ubase;
+  // (at 4096 points with the hint: this pass 529 -> 540 us, the column pass after it
+  // 765-778 -> 736 us)
 #pragma unroll
-  for (int m = 0; m < E; ++m) gst<NT_ROW_ST>(dst + (lo_t + TS * (unsigned)m), v[m]);
+  for (int m = 0; m < E; ++m) gst<N >= 4096 ? NT_TILE_ST : NT_ROW_ST>(dst + (lo_t + TS * (unsigned)m), v[m]);
 }
 
 // Adjoint of a multi-plane stack, one sweep per plane.  For the fields [b0, b1) of a
@@ -779,7 +786,8 @@ __device__ __forceinline__ void rows_adj_chain_body(const PassArgs<T>& a, int b0
         line_fft<T, N, E, false>(v, t2, acc, a.tw + (N - 2));
       LP2D_GLB cx<T>* __restrict__ dst = field_dst(a, b) + ubase;
 #pragma unroll
-      for (int m = 0; m < E; ++m) gst<NT_ROW_ST>(dst + (lo_t + TS * (unsigned)m), v[m]);
+      for (int m = 0; m < E; ++m)
+        gst<N >= 4096 ? NT_TILE_ST : NT_ROW_ST>(dst + (lo_t + TS * (unsigned)m), v[m]);
     }
   }
   if (partial) {
