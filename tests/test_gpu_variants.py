@@ -95,7 +95,7 @@ def test_every_row_variant_with_a_phase_screen(restore_selection, n):
     from lightprop2d_amd import propagation as P
     lib = restore_selection
     area, wl, z, na = OPTICS["pupil"]
-    batch = 3 if n <= 1024 else (2 if n <= 2048 else 1)
+    batch = 3 if n <= 1024 else 2   # per-field screens reach field 1 at every size
     rs = np.random.RandomState(3 * n)
     u = (rs.standard_normal((batch, n, n)) + 1j * rs.standard_normal((batch, n, n))).astype(
         np.complex64)
